@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""bench.py -- P3M particle-updates/s of the CUBE force-and-update step on one MI355X.
+
+Workload (default): 512^3 CDM particles in a 600 Mpc/h box, seeded Zel'dovich
+displacements at z=2 (BBKS P(k), h=0.6817, Om=0.301, Ob=0.048, ns=0.9693,
+sigma8=0.8), int16 compressed coordinates.  One step = drift (re-sort and
+re-encode) + nn=1 buffer layout + kick (PM1 + PM2 + PM3 + PP), the order of
+main.f90:34-38.  Inputs are generated on the GPU and are resident in HBM when
+the timed region starts.  value = particles * steps / seconds.
+
+Prints ONE JSON line (see the driver contract) with a `roofline` object for
+the dominant kernel (HIP-event durations on the library stream) and a
+`cpu_baseline` object: the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "P3M particle-updates/s at 512³ on 1 MI355X (FP32 vs FP64 form); PM fwd+adjoint time"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(ng, precision, steps, warmup, z, box, timing=True):
+    from cube_nu_amd import P3M
+
+    p = P3M(ng, izipx=2, precision=precision, box=box)
+    t0 = time.time()
+    ic = p.make_ic(seed=20260101, z=z, nic=2)
+    log(f"[bench] fp{precision} ng={ng}: IC {ic['nplocal']} particles in {time.time() - t0:.1f}s, vz_max {ic['vz_max']:.3f}")
+    a = 1.0 / (1.0 + z)
+    # step criteria from a zero-length kick (timestep.f90:34 takes their minimum)
+    k0 = p.kick_resident(a_mid=a, dt=0.0)
+    dt = min(k0["dt_pm1"], k0["dt_pm2"], k0["dt_pm3"], k0["dt_pp"], 1.0)
+    vz = ic["vz_max"]
+    log(f"[bench] dt={dt:.4g} (pm1 {k0['dt_pm1']:.3g} pm2 {k0['dt_pm2']:.3g} pm3 {k0['dt_pm3']:.3g} pp {k0['dt_pp']:.3g}) napm={k0['napm']}")
+    for w in range(warmup):
+        d, k = p.step_resident(dt_old=dt, dt=dt, vz_max=vz, a_mid=a)
+        vz = k["vz_max"]
+        log(f"[bench] warmup step {w}: nplocal {d['nplocal']} nlayer {d['nlayer']}")
+    p.synchronize()
+    if timing:
+        p.timing(True)
+        p.timing_reset()
+    t0 = time.perf_counter()
+    nloc = 0
+    for s in range(steps):
+        d, k = p.step_resident(dt_old=dt, dt=dt, vz_max=vz, a_mid=a)
+        vz = k["vz_max"]
+        nloc = d["nplocal"]
+    p.synchronize()
+    t = time.perf_counter() - t0
+    rep = p.timing_report() if timing else {}
+    p.close()
+    log(f"[bench] fp{precision}: {steps} steps in {t:.3f}s ({1e3 * t / steps:.1f} ms/step)")
+    return {"seconds": t, "n": nloc, "report": rep, "dt": dt}
+
+
+def roofline(rep, steps):
+    kern = {k: v for k, v in rep.items() if v["ms"] > 0 and k != "green"}
+    if not kern:
+        return None, {}
+    name = max(kern, key=lambda k: kern[k]["ms"])
+    v = kern[name]
+    avg_ms = v["ms"] / v["calls"]
+    phases = {k: round(vv["ms"] / steps, 3) for k, vv in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
+    if name == "pp_force":
+        pairs = rep.get("pp_pairs_last", {}).get("calls", 0)
+        ach = 40.0 * pairs / (avg_ms * 1e-3) / 1e12 / max(1, v["calls"] // steps)
+        r = {"kernel": name, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+             "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "avg_launch_ms": avg_ms}
+    else:
+        ach = (v["bytes"] / v["calls"]) / (avg_ms * 1e-3) / 1e9
+        r = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+             "bytes_per_launch": v["bytes"] / v["calls"]}
+    return r, phases
+
+
+def cpu_baseline(ng_sample, z, box):
+    """Oracle (C++/OpenMP restatement) timed on the host: one full step
+    (kick + drift) at ng_sample^3 with the same IC recipe."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cube_nu_amd import P3M
+    from inputs import oparams
+    from oracle import pyoracle as orc
+
+    p = P3M(ng_sample, izipx=2, box=box * ng_sample / 512.0)
+    p.make_ic(seed=20260101, z=z, nic=2)
+    xp, vp, pid, rhoc = p.download()
+    p.close()
+    P = oparams(p.geo)
+    tfs = orc.tf_default(P)
+    a = 1.0 / (1.0 + z)
+    v = vp.copy()
+    out, _, _ = orc.kick(P, xp, v, rhoc, tfs, a, 0.0)
+    dt = min(out["dt_pm1"], out["dt_pm2"], out["dt_pm3"], out["dt_pp"], 1.0)
+    t0 = time.perf_counter()
+    out, _, _ = orc.kick(P, xp, v, rhoc, tfs, a, dt)
+    st, *_ = orc.drift(P, xp, v, pid, rhoc, dt, dt, out["vz_max"])
+    t = time.perf_counter() - t0
+    n = int(rhoc[:, :, :, 4:-4, 4:-4, 4:-4].sum())
+    return {"value": n / t, "unit": "particle-updates/s", "cores": orc.num_threads(), "kind": "port",
+            "sample": f"one full oracle step (kick PM1+PM2+PM3+PP + drift) at {ng_sample}^3 = {n} particles, "
+                      f"same Zel'dovich recipe, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--z", type=float, default=2.0)
+    ap.add_argument("--box", type=float, default=600.0)
+    ap.add_argument("--fp64-steps", type=int, default=1, help="steps of the plain FP64 form (0 = skip)")
+    ap.add_argument("--cpu-sample-ng", type=int, default=128, help="0 = skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it (see DESIGN.md, sharding)"}))
+        sys.exit(2)
+    r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box)
+    value = r32["n"] * args.steps / r32["seconds"]
+    roof, phases = roofline(r32["report"], args.steps)
+    fp64 = None
+    if args.fp64_steps > 0:
+        try:
+            r64 = run_form(args.ng, 64, args.fp64_steps, 0, args.z, args.box, timing=False)
+            fp64 = {"value": r64["n"] * args.fp64_steps / r64["seconds"],
+                    "ms_per_step": 1e3 * r64["seconds"] / args.fp64_steps,
+                    "fp32_speedup": value / (r64["n"] * args.fp64_steps / r64["seconds"])}
+        except Exception as e:  # reported, not hidden
+            fp64 = {"error": str(e)}
+    cpu = None
+    if args.cpu_sample_ng > 0:
+        try:
+            cpu = cpu_baseline(args.cpu_sample_ng, args.z, args.box)
+        except Exception as e:
+            cpu = {"error": str(e)}
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded Zel'dovich IC (BBKS P(k), sigma8=0.8) generated on the GPU",
+        "config": {"workload": f"{args.ng}^3 CDM, box {args.box:g} Mpc/h, z={args.z:g}, full P3M step "
+                               "(drift + buffer + PM1/PM2/PM3/PP kick), int16 coordinates",
+                   "ng": args.ng, "izipx": 2, "dt": r32["dt"]},
+        "roofline": roof, "cpu_baseline": cpu, "fp64_form": fp64, "phase_ms_per_step": phases,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

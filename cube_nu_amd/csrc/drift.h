@@ -1,0 +1,53 @@
+// drift.h -- drift with re-sort/re-encode (drift.f90:1-233) and the nn=1
+// buffer layout (buffer_grid.f90:1-226, buffer_particle.f90:1-375).
+#pragma once
+#include "common.h"
+
+namespace cube {
+
+constexpr int kMaxShift = 16;  // largest cell shift the offset pass orders
+
+struct DriftParams {
+  float dt_mid;
+  int nlayer;
+  int nt, nte, ncb, nnt, rcs, izipx;
+  int M[3];       // maximum |cell shift| per axis
+  int D[3];       // 2M+1
+};
+
+struct DriftScratch {
+  int32_t* flags = nullptr;     // [0] error bits, [1..3] max shift per axis
+  int32_t* rank = nullptr;      // per source particle
+  int32_t* table = nullptr;     // per source cell per direction (count -> offset)
+  int64_t table_cap = 0;        // int32 elements
+  int32_t* newcount = nullptr;  // ntiles * nt^3 physical destination counts
+  int64_t* base = nullptr;      // ntiles * nt^3 + 1 (scan of newcount)
+  int64_t* tile_tot = nullptr;  // per tile physical total
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+
+// Error bits written to flags[0].
+enum { DRIFT_OUT_OF_RANGE = 1, DRIFT_PID_ZERO = 2 };
+
+// Pass 1: destination range check and maximum shift.
+void drift_shift(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
+                 int64_t ncells, DriftScratch& S, hipStream_t s);
+// Passes 2-5 for tiles [t0, t1): rank, offsets, scatter (base scan done by caller).
+void drift_rank(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
+                int t0, int t1, DriftScratch& S, hipStream_t s);
+void drift_offsets(const DriftParams& P, int t0, int t1, DriftScratch& S, hipStream_t s);
+void drift_scatter(const DriftParams& P, const void* xp, const float* vp, const int64_t* pid, const int32_t* rhoc,
+                   const int64_t* off, int t0, int t1, DriftScratch& S, void* xp_out, float* vp_out,
+                   int64_t* pid_out, hipStream_t s);
+// rhoc with zero buffers and physical counts from newcount.
+void drift_write_rhoc(const DriftParams& P, const DriftScratch& S, int32_t* rhoc_out, hipStream_t s);
+
+// Buffer: physical layout (counts in rhoc_phys, extended array with zero
+// buffers) -> extended layout.
+void buffer_rhoc(const Geo& g, const int32_t* rhoc_phys, int32_t* rhoc_ext, int32_t* physcount, hipStream_t s);
+void buffer_copy(const Geo& g, const int32_t* rhoc_ext, const int64_t* ext_off, const int64_t* phys_off,
+                 const void* xp, const float* vp, const int64_t* pid, void* xp_out, float* vp_out, int64_t* pid_out,
+                 hipStream_t s);
+
+}  // namespace cube

@@ -1,0 +1,355 @@
+// drift.hip -- drift.f90 on the GPU, bit-exact.
+//
+// The reference fixes the order of particles inside a destination cell by
+// its plane-colour OpenMP passes (drift.f90:46-49, 96-136): source planes are
+// visited in passes ilayer = (k-(1-ncb)) mod nlayer, each plane by one thread
+// in (j, i, l) order.  With the colour count nlayer = 2*ceil(dt_mid*vz_max/4)
+// + 2, a destination cell receives from at most one plane per pass, so the
+// order is the key (pass(k_src), k_src, j_src, i_src, l).
+//
+// GPU form (no atomics on any order):
+//  1. shift pass: destination of every particle, range check, maximum cell
+//     shift per axis, per-destination integer counts (atomic adds of
+//     integers are order-independent);
+//  2. rank pass (one wavefront per source cell): for each destination
+//     direction d, the rank of each particle among the cell's earlier
+//     particles with the same d (ballot + popcount), and the per-(cell, d)
+//     count;
+//  3. offset pass (one thread per destination cell): walks its neighbouring
+//     source cells in key order and turns the (cell, d) counts into offsets;
+//  4. scatter pass: out = base(dest) + offset(cell, d) + rank, with the
+//     re-encode xp' = xp + nint(dt_mid*v/(x_resolution*ratio_cs)) wrapped to
+//     izipx bytes (drift.f90:117).
+#include "drift.h"
+
+namespace cube {
+namespace {
+
+template <int IZ>
+using XT = typename XP<IZ>::T;
+
+}  // namespace
+
+namespace {
+
+template <int IZ>
+__device__ inline void dest_cell(const DriftParams& P, const XT<IZ>* xp, const float* vp, int64_t ip,
+                                 const int ijk[3], int g[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const float wv = P.dt_mid * vp[3 * ip + d];
+    const double dx = (double)(wv / (float)P.rcs);               // drift.f90:57
+    const double xq = ((double)ijk[d] - 1.0) + XP<IZ>::off(xp, ip, d);  // drift.f90:59
+    g[d] = (int)ceil(xq + dx);                                   // drift.f90:63
+  }
+}
+
+__device__ inline void cell_of(const DriftParams& P, int64_t cglob, int* T, int ijk[3]) {
+  const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
+  *T = (int)(cglob / nct);
+  const int e = (int)(cglob % nct);
+  ijk[0] = e % P.nte + 1 - P.ncb;
+  ijk[1] = (e / P.nte) % P.nte + 1 - P.ncb;
+  ijk[2] = e / (P.nte * P.nte) + 1 - P.ncb;
+}
+
+__device__ inline bool physical(const DriftParams& P, const int g[3]) {
+  return g[0] >= 1 && g[0] <= P.nt && g[1] >= 1 && g[1] <= P.nt && g[2] >= 1 && g[2] <= P.nt;
+}
+
+__device__ inline int dcode(const DriftParams& P, const int g[3], const int s[3]) {
+  return ((g[2] - s[2] + P.M[2]) * P.D[1] + (g[1] - s[1] + P.M[1])) * P.D[0] + (g[0] - s[0] + P.M[0]);
+}
+
+__device__ inline int64_t pcell(const DriftParams& P, int T, const int g[3]) {
+  return (((int64_t)T * P.nt + (g[2] - 1)) * P.nt + (g[1] - 1)) * P.nt + (g[0] - 1);
+}
+
+template <int IZ>
+__global__ void k_drift_shift(DriftParams P, const XT<IZ>* __restrict__ xp, const float* __restrict__ vp,
+                              const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off, int64_t ncells,
+                              int32_t* __restrict__ flags, int32_t* __restrict__ newcount) {
+  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (c >= ncells) return;
+  int T, ijk[3];
+  cell_of(P, c, &T, ijk);
+  const int cnt = rhoc[c];
+  const int64_t base = off[c];
+  const int lo = 1 - 2 * P.ncb, hi = P.nt + 2 * P.ncb;
+  int err = 0, ms[3] = {0, 0, 0};
+  for (int l = lane; l < cnt; l += 64) {
+    int g[3];
+    dest_cell<IZ>(P, xp, vp, base + l, ijk, g);
+    for (int d = 0; d < 3; ++d)
+      if (g[d] < lo || g[d] > hi) err = 1;
+    if (physical(P, g)) {
+      for (int d = 0; d < 3; ++d) ms[d] = max(ms[d], abs(g[d] - ijk[d]));
+      atomicAdd(&newcount[pcell(P, T, g)], 1);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    err |= __shfl_down(err, o, 64);
+    for (int d = 0; d < 3; ++d) ms[d] = max(ms[d], __shfl_down(ms[d], o, 64));
+  }
+  if (lane == 0 && cnt > 0) {
+    if (err) atomicOr(&flags[0], DRIFT_OUT_OF_RANGE);
+    for (int d = 0; d < 3; ++d) atomicMax(&flags[1 + d], ms[d]);
+  }
+}
+
+template <int IZ>
+__global__ void k_drift_rank(DriftParams P, const XT<IZ>* __restrict__ xp, const float* __restrict__ vp,
+                             const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off, int64_t c0,
+                             int64_t ncells, int32_t* __restrict__ rank, int32_t* __restrict__ table) {
+  const int64_t cl = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (cl >= ncells) return;
+  const int64_t c = c0 + cl;
+  int T, ijk[3];
+  cell_of(P, c, &T, ijk);
+  const int cnt = rhoc[c];
+  const int64_t base = off[c];
+  const int Dt = P.D[0] * P.D[1] * P.D[2];
+  int32_t* tbl = table + cl * Dt;
+  for (int l0 = 0; l0 < cnt; l0 += 64) {
+    const int l = l0 + lane;
+    int key = -1;
+    if (l < cnt) {
+      int g[3];
+      dest_cell<IZ>(P, xp, vp, base + l, ijk, g);
+      if (physical(P, g)) key = dcode(P, g, ijk);
+    }
+    unsigned long long active = __ballot(key >= 0);
+    int myrank = -1;
+    while (active) {
+      const int leader = __ffsll((long long)active) - 1;
+      const int lk = __shfl(key, leader, 64);
+      const unsigned long long peers = __ballot(key == lk);
+      int bc = 0;
+      if (lane == leader) bc = tbl[lk];
+      bc = __shfl(bc, leader, 64);
+      if (key == lk) myrank = bc + __popcll(peers & ((1ull << lane) - 1ull));
+      if (lane == leader) tbl[lk] = bc + __popcll(peers);
+      active &= ~peers;
+    }
+    if (l < cnt) rank[base + l] = myrank;
+  }
+}
+
+__global__ void k_drift_offsets(DriftParams P, int t0, int t1, int32_t* __restrict__ table,
+                                const int32_t* __restrict__ newcount, int32_t* __restrict__ flags) {
+  const int64_t nt3 = (int64_t)P.nt * P.nt * P.nt;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)(t1 - t0) * nt3) return;
+  const int T = t0 + (int)(t / nt3);
+  const int q = (int)(t % nt3);
+  const int g[3] = {q % P.nt + 1, (q / P.nt) % P.nt + 1, q / (P.nt * P.nt) + 1};
+  const int lo = 1 - P.ncb, hi = P.nt + P.ncb;
+  const int Dt = P.D[0] * P.D[1] * P.D[2];
+  // source planes in pass order (pass(k) = (k - (1-ncb)) mod nlayer), then k
+  int ks[2 * kMaxShift + 1];
+  int nk = 0;
+  for (int k = max(lo, g[2] - P.M[2]); k <= min(hi, g[2] + P.M[2]); ++k) ks[nk++] = k;
+  for (int a = 1; a < nk; ++a) {
+    int v = ks[a], pv = (v - lo) % P.nlayer, b = a - 1;
+    while (b >= 0) {
+      int pb = (ks[b] - lo) % P.nlayer;
+      if (pb > pv || (pb == pv && ks[b] > v)) {
+        ks[b + 1] = ks[b];
+        --b;
+      } else
+        break;
+    }
+    ks[b + 1] = v;
+  }
+  const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
+  int running = 0;
+  for (int a = 0; a < nk; ++a) {
+    const int k = ks[a];
+    for (int j = max(lo, g[1] - P.M[1]); j <= min(hi, g[1] + P.M[1]); ++j)
+      for (int i = max(lo, g[0] - P.M[0]); i <= min(hi, g[0] + P.M[0]); ++i) {
+        const int s[3] = {i, j, k};
+        const int64_t cl = (int64_t)(T - t0) * nct + rc_index(i, j, k, 0, P.nte, P.ncb);
+        const int64_t idx = cl * Dt + dcode(P, g, s);
+        const int c = table[idx];
+        table[idx] = running;
+        running += c;
+      }
+  }
+  if (running != newcount[pcell(P, T, g)]) atomicOr(&flags[0], 4);
+}
+
+template <int IZ>
+__global__ void k_drift_scatter(DriftParams P, const XT<IZ>* __restrict__ xp, const float* __restrict__ vp,
+                                const int64_t* __restrict__ pid, const int32_t* __restrict__ rhoc,
+                                const int64_t* __restrict__ off, int64_t c0, int64_t ncells,
+                                const int32_t* __restrict__ rank, const int32_t* __restrict__ table,
+                                const int64_t* __restrict__ base_out, XT<IZ>* __restrict__ xo,
+                                float* __restrict__ vo, int64_t* __restrict__ po, int32_t* __restrict__ flags) {
+  const int64_t cl = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (cl >= ncells) return;
+  const int64_t c = c0 + cl;
+  int T, ijk[3];
+  cell_of(P, c, &T, ijk);
+  const int cnt = rhoc[c];
+  const int64_t base = off[c];
+  const int Dt = P.D[0] * P.D[1] * P.D[2];
+  const double xr = 1.0 / (double)(1 << XP<IZ>::B);
+  for (int l = lane; l < cnt; l += 64) {
+    const int64_t ip = base + l;
+    const int r = rank[ip];
+    if (r < 0) continue;
+    int g[3];
+    dest_cell<IZ>(P, xp, vp, ip, ijk, g);
+    const int64_t pos = base_out[pcell(P, T, g)] + table[cl * Dt + dcode(P, g, ijk)] + r;
+    for (int d = 0; d < 3; ++d) {
+      const float wv = P.dt_mid * vp[3 * ip + d];
+      const double sh = (double)wv / (xr * (double)P.rcs);
+      const long long dn = (long long)round(sh);  // nint: half away from zero
+      const long long nv = (long long)xp[3 * ip + d] + dn;
+      if (IZ == 2) xo[3 * pos + d] = (int16_t)(uint16_t)(nv & 0xffff);
+      else xo[3 * pos + d] = (int8_t)(uint8_t)(nv & 0xff);
+      vo[3 * pos + d] = vp[3 * ip + d];
+    }
+    const int64_t pv = pid ? pid[ip] : 1;
+    if (po) po[pos] = pv;
+    if (pv <= 0) atomicOr(&flags[0], DRIFT_PID_ZERO);
+  }
+}
+
+__global__ void k_drift_rhoc(DriftParams P, int64_t ncells, const int32_t* __restrict__ newcount,
+                             int32_t* __restrict__ rhoc) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int T, ijk[3];
+  cell_of(P, c, &T, ijk);
+  rhoc[c] = physical(P, ijk) ? newcount[pcell(P, T, ijk)] : 0;
+}
+
+__global__ void k_buffer_rhoc(Geo g, const int32_t* __restrict__ rp, int32_t* __restrict__ re,
+                              int32_t* __restrict__ physcount) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncells_ext) return;
+  const int64_t T = c / g.ncell_tile, e = c % g.ncell_tile;
+  const int64_t it[3] = {T % g.nnt, (T / g.nnt) % g.nnt, T / (g.nnt * g.nnt)};
+  const int64_t ijk[3] = {e % g.nte + 1 - g.ncb, (e / g.nte) % g.nte + 1 - g.ncb, e / (g.nte * g.nte) + 1 - g.ncb};
+  int64_t st[3], sc[3];
+  for (int d = 0; d < 3; ++d) {
+    int64_t gc = it[d] * g.nt + ijk[d] - 1;
+    gc = ((gc % g.nc) + g.nc) % g.nc;
+    st[d] = gc / g.nt;
+    sc[d] = gc % g.nt + 1;
+  }
+  const int64_t Ts = (st[2] * g.nnt + st[1]) * g.nnt + st[0];
+  const int32_t v = rp[rc_index(sc[0], sc[1], sc[2], Ts, g.nte, g.ncb)];
+  re[c] = v;
+  const bool phys = ijk[0] >= 1 && ijk[0] <= g.nt && ijk[1] >= 1 && ijk[1] <= g.nt && ijk[2] >= 1 && ijk[2] <= g.nt;
+  if (phys) physcount[((T * g.nt + ijk[2] - 1) * g.nt + ijk[1] - 1) * g.nt + ijk[0] - 1] = v;
+}
+
+__global__ void k_buffer_copy(Geo g, const int32_t* __restrict__ re, const int64_t* __restrict__ eoff,
+                              const int64_t* __restrict__ poff, const char* __restrict__ xp,
+                              const float* __restrict__ vp, const int64_t* __restrict__ pid, char* __restrict__ xo,
+                              float* __restrict__ vo, int64_t* __restrict__ po) {
+  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (c >= g.ncells_ext) return;
+  const int cnt = re[c];
+  if (cnt == 0) return;
+  const int64_t T = c / g.ncell_tile, e = c % g.ncell_tile;
+  const int64_t it[3] = {T % g.nnt, (T / g.nnt) % g.nnt, T / (g.nnt * g.nnt)};
+  const int64_t ijk[3] = {e % g.nte + 1 - g.ncb, (e / g.nte) % g.nte + 1 - g.ncb, e / (g.nte * g.nte) + 1 - g.ncb};
+  int64_t st[3], sc[3];
+  for (int d = 0; d < 3; ++d) {
+    int64_t gc = it[d] * g.nt + ijk[d] - 1;
+    gc = ((gc % g.nc) + g.nc) % g.nc;
+    st[d] = gc / g.nt;
+    sc[d] = gc % g.nt + 1;
+  }
+  const int64_t Ts = (st[2] * g.nnt + st[1]) * g.nnt + st[0];
+  const int64_t src = poff[((Ts * g.nt + sc[2] - 1) * g.nt + sc[1] - 1) * g.nt + sc[0] - 1];
+  const int64_t dst = eoff[c];
+  const int esz = (int)g.izipx;
+  for (int q = lane; q < 3 * esz * cnt; q += 64) xo[3 * esz * dst + q] = xp[3 * esz * src + q];
+  for (int q = lane; q < 3 * cnt; q += 64) vo[3 * dst + q] = vp[3 * src + q];
+  if (pid)
+    for (int q = lane; q < cnt; q += 64) po[dst + q] = pid[src + q];
+}
+
+}  // namespace
+
+void drift_shift(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
+                 int64_t ncells, DriftScratch& S, hipStream_t s) {
+  const int64_t nthr = ncells * 64;
+  if (P.izipx == 2)
+    hipLaunchKernelGGL(k_drift_shift<2>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P,
+                       (const int16_t*)xp, vp, rhoc, off, ncells, S.flags, S.newcount);
+  else
+    hipLaunchKernelGGL(k_drift_shift<1>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P, (const int8_t*)xp,
+                       vp, rhoc, off, ncells, S.flags, S.newcount);
+  CUBE_HIP(hipGetLastError());
+}
+
+void drift_rank(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
+                int t0, int t1, DriftScratch& S, hipStream_t s) {
+  const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
+  const int64_t ncells = (int64_t)(t1 - t0) * nct;
+  const int64_t nthr = ncells * 64;
+  if (P.izipx == 2)
+    hipLaunchKernelGGL(k_drift_rank<2>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P, (const int16_t*)xp,
+                       vp, rhoc, off, (int64_t)t0 * nct, ncells, S.rank, S.table);
+  else
+    hipLaunchKernelGGL(k_drift_rank<1>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P, (const int8_t*)xp,
+                       vp, rhoc, off, (int64_t)t0 * nct, ncells, S.rank, S.table);
+  CUBE_HIP(hipGetLastError());
+}
+
+void drift_offsets(const DriftParams& P, int t0, int t1, DriftScratch& S, hipStream_t s) {
+  const int64_t n = (int64_t)(t1 - t0) * P.nt * P.nt * P.nt;
+  hipLaunchKernelGGL(k_drift_offsets, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, P, t0, t1, S.table,
+                     S.newcount, S.flags);
+  CUBE_HIP(hipGetLastError());
+}
+
+void drift_scatter(const DriftParams& P, const void* xp, const float* vp, const int64_t* pid, const int32_t* rhoc,
+                   const int64_t* off, int t0, int t1, DriftScratch& S, void* xp_out, float* vp_out,
+                   int64_t* pid_out, hipStream_t s) {
+  const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
+  const int64_t ncells = (int64_t)(t1 - t0) * nct;
+  const int64_t nthr = ncells * 64;
+  if (P.izipx == 2)
+    hipLaunchKernelGGL(k_drift_scatter<2>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P,
+                       (const int16_t*)xp, vp, pid, rhoc, off, (int64_t)t0 * nct, ncells, S.rank, S.table, S.base,
+                       (int16_t*)xp_out, vp_out, pid_out, S.flags);
+  else
+    hipLaunchKernelGGL(k_drift_scatter<1>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P,
+                       (const int8_t*)xp, vp, pid, rhoc, off, (int64_t)t0 * nct, ncells, S.rank, S.table, S.base,
+                       (int8_t*)xp_out, vp_out, pid_out, S.flags);
+  CUBE_HIP(hipGetLastError());
+}
+
+void drift_write_rhoc(const DriftParams& P, const DriftScratch& S, int32_t* rhoc_out, hipStream_t s) {
+  const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
+  const int64_t ncells = (int64_t)P.nnt * P.nnt * P.nnt * nct;
+  hipLaunchKernelGGL(k_drift_rhoc, dim3((unsigned)ceil_div(ncells, 256)), dim3(256), 0, s, P, ncells, S.newcount,
+                     rhoc_out);
+  CUBE_HIP(hipGetLastError());
+}
+
+void buffer_rhoc(const Geo& g, const int32_t* rhoc_phys, int32_t* rhoc_ext, int32_t* physcount, hipStream_t s) {
+  hipLaunchKernelGGL(k_buffer_rhoc, dim3((unsigned)ceil_div(g.ncells_ext, 256)), dim3(256), 0, s, g, rhoc_phys,
+                     rhoc_ext, physcount);
+  CUBE_HIP(hipGetLastError());
+}
+
+void buffer_copy(const Geo& g, const int32_t* rhoc_ext, const int64_t* ext_off, const int64_t* phys_off,
+                 const void* xp, const float* vp, const int64_t* pid, void* xp_out, float* vp_out, int64_t* pid_out,
+                 hipStream_t s) {
+  const int64_t nthr = g.ncells_ext * 64;
+  hipLaunchKernelGGL(k_buffer_copy, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, g, rhoc_ext, ext_off,
+                     phys_off, (const char*)xp, vp, pid, (char*)xp_out, vp_out, pid_out);
+  CUBE_HIP(hipGetLastError());
+}
+
+}  // namespace cube

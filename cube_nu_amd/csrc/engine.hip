@@ -1,0 +1,920 @@
+// engine.hip -- context, level pipelines and the C-ABI of cube_p3m.h.
+//
+// One context = one image (nn=1) on the current HIP device, one stream.
+// The kick runs PM1 (coarse periodic mesh), PM2 (64 tile meshes, batched),
+// PM3 (sub-tile meshes grouped by resolution, batched) and PP, in the order
+// of kick.f90:32-311; the drift and the nn=1 buffer layout complete a step.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "../../include/cube_p3m.h"
+#include "common.h"
+#include "drift.h"
+#include "fft.h"
+#include "ic.h"
+#include "pm.h"
+#include "pp.h"
+#include "timing.h"
+
+using namespace cube;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  T* ensure(size_t m) {
+    if (m > n) {
+      release();
+      if (m) {
+        hipError_t e = hipMalloc(&p, m * sizeof(T));
+        if (e != hipSuccess) {
+          p = nullptr;
+          n = 0;
+          throw Error(CUBE_ERR_HIP, "hipMalloc of " + std::to_string(m * sizeof(T)) + " bytes failed: " +
+                                        hipGetErrorString(e));
+        }
+      }
+      n = m;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+// G_grid = 1.0/6.0/pi (variables.f90:28) as stored in real(4).
+inline float g_grid() { return (float)((double)(1.0f / 6.0f) / kPi); }
+// dt = 0.5*sqrt(1./(sqrt(f2max)*a_mid*G_grid)) (kick.f90:39, 69, 239), or
+// with 0.05 for PP (kick.f90:298), evaluated in real(4).
+inline float dt_from_f2(float c, float f2, float a_mid) {
+  return c * std::sqrt(1.0f / ((std::sqrt(f2) * a_mid) * g_grid()));
+}
+
+}  // namespace
+
+struct cube_p3m_ctx {
+  cube_p3m_config cfg{};
+  Geo g{};
+  int prec = 32;
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::string err;
+  KTimer tm;
+
+  // transfer functions: [1]=tf1 (nc), [2]=tf2 (ngt), [3..6]=tf3_2..tf3_8
+  DevBuf<float> tf[7];
+  int tfn[7] = {0, 0, 0, 0, 0, 0, 0};
+  FFTCube<float> fcf[7];
+  FFTCube<double> fcd[7];
+
+  // resident particle state (double-buffered)
+  DevBuf<char> xp[2];
+  DevBuf<float> vp[2];
+  DevBuf<int64_t> pid[2];
+  int cur = 0;
+
+  int layout = 0;  // 0 none, 1 extended (buffered), 2 physical (after drift)
+  int64_t np = 0;
+  bool has_pid = false;
+  DevBuf<int32_t> rhoc[2];
+  int rcur = 0;
+  DevBuf<int64_t> off;
+  bool off_valid = false;
+
+  // scratch
+  DevBuf<char> scan_tmp;
+  DevBuf<char> work;
+  DevBuf<PMBox> boxes;
+  DevBuf<PPBox> ppboxes;
+  DevBuf<PMOut> pmout;
+  DevBuf<PPOut> ppout;
+  DevBuf<int32_t> ires, napm;
+  DevBuf<int32_t> ppcount;
+  DevBuf<int64_t> ppstart;
+  DevBuf<float4> ppent;
+  DevBuf<int32_t> dflags, drank, dtable, dnewcount;
+  DevBuf<int64_t> dbase;
+  DevBuf<int32_t> physcount;
+  DevBuf<int64_t> physoff;
+  int64_t last_pairs = 0;
+
+  ~cube_p3m_ctx() {
+    for (int i = 0; i < 7; ++i) {
+      fcf[i].release();
+      fcd[i].release();
+    }
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  int esz() const { return (int)g.izipx; }
+  void* xpc() { return xp[cur].p; }
+  float* vpc() { return vp[cur].p; }
+  int64_t* pidc() { return has_pid ? pid[cur].p : nullptr; }
+  int32_t* rhocc() { return rhoc[rcur].p; }
+
+  void* scan_scratch(int64_t n) {
+    size_t b = scan_tmp_bytes(n);
+    return scan_tmp.ensure(b);
+  }
+  void scan(const int32_t* in, int64_t* out, int64_t n) {
+    void* t = scan_scratch(n);
+    scan_counts(in, out, n, t, scan_tmp.n, s);
+  }
+  // Grow one particle buffer set; growing discards its contents, so callers
+  // only grow the set they are about to overwrite.
+  int64_t capb[2] = {0, 0};
+  void ensure_set(int b, int64_t c) {
+    if (c > capb[b]) {
+      xp[b].ensure((size_t)c * 3 * esz());
+      vp[b].ensure((size_t)c * 3);
+      pid[b].ensure((size_t)c);
+      capb[b] = c;
+    }
+    rhoc[0].ensure(g.ncells_ext);
+    rhoc[1].ensure(g.ncells_ext);
+    off.ensure(g.ncells_ext + 1);
+  }
+  void ensure_offsets() {
+    if (off_valid) return;
+    KScope k(&tm, "spine_scan", (double)g.ncells_ext * 12);
+    scan(rhocc(), off.p, g.ncells_ext);
+    off_valid = true;
+  }
+  int64_t ws_budget() const {
+    if (cfg.workspace_bytes > 0) return cfg.workspace_bytes;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (int64_t)8 << 30;
+    return std::max<int64_t>((int64_t)256 << 20, std::min<int64_t>((int64_t)24 << 30, (int64_t)(fr * 0.35)));
+  }
+
+  void init();
+  void set_tf_default();
+  template <typename R>
+  void kick(const cube_kick_in& in, cube_kick_out& out, float* density_host);
+  template <typename R>
+  void run_level(int level, const std::vector<PMBox>& bx, const float* G, const FFTCube<R>& fc, const PMLevel& L,
+                 const cube_kick_in& in, PMOut* o, const char* tag, float* density_host = nullptr);
+  void pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_in& in, PPOut* o);
+  void drift(const cube_drift_in& in, cube_drift_out& out);
+  void buffer(int64_t* np_ext);
+};
+
+// ---------------------------------------------------------------------------
+void cube_p3m_ctx::init() {
+  const cube_p3m_config& c = cfg;
+  if (c.ng <= 0 || c.nnt <= 0 || c.nns <= 0 || c.ratio_cs <= 0 || c.ncb <= 0 || c.ngb < 0)
+    throw Error(CUBE_ERR_ARG, "non-positive size in configuration");
+  if (c.izipx != 1 && c.izipx != 2) throw Error(CUBE_ERR_ARG, "izipx must be 1 or 2");
+  if (c.precision != 32 && c.precision != 64) throw Error(CUBE_ERR_ARG, "precision must be 32 or 64");
+  if (c.ng % (c.ratio_cs * c.nnt * c.nns) != 0) throw Error(CUBE_ERR_ARG, "ng must be a multiple of ratio_cs*nnt*nns");
+  prec = c.precision;
+  g.init(c.ng, c.nnt, c.nns, c.ratio_cs, c.ncb, c.ngb, c.izipx, c.image_buffer, c.tile_buffer);
+  if (g.ngb < 4) throw Error(CUBE_ERR_ARG, "ngb must be >= 4 (gradient stencil)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(CUBE_ERR_NO_DEVICE, "no HIP device visible");
+  CUBE_HIP(hipGetDevice(&device));
+  CUBE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  tm.stream = s;
+  const int sizes[7] = {0, (int)g.nc, (int)g.ngt, (int)g.nft[2], (int)g.nft[3], (int)g.nft[4], (int)g.nft[5]};
+  for (int l = 1; l <= 6; ++l) {
+    tfn[l] = sizes[l];
+    if (prec == 32) fcf[l].init(sizes[l]);
+    else fcd[l].init(sizes[l]);
+    tf[l].ensure((size_t)(sizes[l] / 2 + 1) * sizes[l] * sizes[l]);
+  }
+  pmout.ensure(4);
+  ppout.ensure(1);
+  napm.ensure(8);
+  ires.ensure(g.ns3);
+  dflags.ensure(8);
+  set_tf_default();
+}
+
+// tf* = Gk* or (1-f_nu)*Gk* (initialize.f90:248-334).
+void cube_p3m_ctx::set_tf_default() {
+  const double fac = cfg.mass_nu > 0 ? (1.0 - cfg.f_nu) : 1.0;
+  KScope k(&tm, "green", 0);
+  green_device(tfn[1], cfg.apm1c, 0.0f, (float)cfg.ratio_cs, fac, tf[1].p, s);  // Green.f90 call at initialize.f90:248
+  green_device(tfn[2], cfg.apm2, cfg.apm1c * (float)cfg.ratio_cs, 1.0f, fac, tf[2].p, s);  // :254
+  for (int iapm = 2; iapm <= 5; ++iapm) {
+    float sf = (float)kRatioSF[iapm];
+    green_device(tfn[iapm + 1], cfg.apm3f, cfg.apm2 * sf, 1.0f / sf, fac, tf[iapm + 1].p, s);  // :261-280
+  }
+  CUBE_HIP(hipStreamSynchronize(s));
+}
+
+template <typename R>
+void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const float* G, const FFTCube<R>& fc,
+                             const PMLevel& L0, const cube_kick_in& in, PMOut* o, const char* tag,
+                             float* density_host) {
+  using C = typename Cplx<R>::T;
+  if (bx_in.empty()) return;
+  const int64_t n = L0.n;
+  const int64_t nreal = n * n * n, ncplx = n * n * (n / 2 + 1);
+  const int64_t per = nreal * (int64_t)sizeof(R) + ncplx * (int64_t)sizeof(C);
+  std::string t(tag);
+  // The coarse level is one periodic mesh shared by all tiles; tile and
+  // sub-tile levels have one mesh per box, batched to fit the workspace.
+  const int64_t nmesh_total = (level == 1) ? 1 : (int64_t)bx_in.size();
+  const int64_t batch = (level == 1) ? 1 : std::max<int64_t>(1, std::min<int64_t>(nmesh_total, ws_budget() / per));
+  work.ensure((size_t)(batch * per));
+  R* mesh = (R*)work.p;
+  C* cplx = (C*)(work.p + batch * nreal * sizeof(R));
+  const size_t bchunk = (level == 1) ? bx_in.size() : (size_t)batch;
+  boxes.ensure(std::max<size_t>(boxes.n, bchunk));
+  for (size_t b0 = 0; b0 < bx_in.size(); b0 += bchunk) {
+    const int nb = (int)std::min<size_t>(bchunk, bx_in.size() - b0);
+    std::vector<PMBox> bx(bx_in.begin() + b0, bx_in.begin() + b0 + nb);
+    if (level != 1)
+      for (int i = 0; i < nb; ++i) bx[i].moff = (int64_t)i * nreal;
+    CUBE_HIP(hipMemcpyAsync(boxes.p, bx.data(), sizeof(PMBox) * nb, hipMemcpyHostToDevice, s));
+    PMLevel L = L0;
+    L.bstride = nreal;
+    const int nmesh = (level == 1) ? 1 : nb;
+    const double cells = (double)nreal * nmesh;
+    {
+      KScope k(&tm, (t + "deposit").c_str(), cells * sizeof(R));
+      pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, o, s);
+    }
+    if (density_host) {
+      std::vector<R> tmp(nreal);
+      CUBE_HIP(hipMemcpyAsync(tmp.data(), mesh, sizeof(R) * nreal, hipMemcpyDeviceToHost, s));
+      CUBE_HIP(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < nreal; ++i) density_host[i] = (float)tmp[i];
+    }
+    const R n3 = (R)((float)n * (float)n * (float)n);
+    if (level == 1) fft_convolve<R>(fc, mesh, cplx, G, 1, (R)0, (R)(float)n, s, &tm, tag);  // pencil_fft.f90:77
+    else fft_convolve<R>(fc, mesh, cplx, G, nmesh, n3, (R)0, s, &tm, tag);                // kick.f90:846
+    {
+      KScope k(&tm, (t + "interp_kick").c_str(), cells * sizeof(R));
+      pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, in.a_mid, in.dt, o, s);
+    }
+  }
+}
+
+void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_in& in, PPOut* o) {
+  // mass_scale and v_scale as kick.f90:569-574
+  float mass_scale;
+  if (cfg.pp_corr && cfg.mass_nu > 0)
+    mass_scale = (float)((double)in.mass_p_cdm * ((1.0 - cfg.f_nu) * (1.0 - cfg.f_nu)));
+  else
+    mass_scale = in.mass_p_cdm;
+  const float v_scale = (float)((double)((in.a_mid * in.dt) / 6.0f) / kPi);
+  const int64_t nns = g.nns;
+  for (int iapm = 2; iapm <= 5; ++iapm) {
+    std::vector<PPBox> bx;
+    for (int64_t it = 0; it < g.ns3; ++it)
+      if (ires_h[it] == iapm) {
+        PPBox b;
+        b.ifs[0] = (int)(it % nns);
+        b.ifs[1] = (int)((it / nns) % nns);
+        b.ifs[2] = (int)((it / (nns * nns)) % nns);
+        b.T = (int)(it / (nns * nns * nns));
+        bx.push_back(b);
+      }
+    if (bx.empty()) continue;
+    PPLevel L;
+    L.sf = kRatioSF[iapm];
+    L.npgrid = (int)g.ntt * L.sf;
+    L.ntt = (int)g.ntt;
+    L.nte = (int)g.nte;
+    L.ncb = (int)g.ncb;
+    L.rcs = (int)g.rcs;
+    L.pp_range = 3.5f / (float)L.sf;  // apm3 (parameters.f90:43)
+    L.pp_range2 = L.pp_range * L.pp_range;
+    L.app = cfg.app;
+    L.mass_scale = mass_scale;
+    L.v_scale = v_scale;
+    const int64_t m = L.npgrid + 2, m3 = m * m * m;
+    const int64_t budget = ws_budget();
+    int64_t batch = std::max<int64_t>(1, std::min<int64_t>((int64_t)bx.size(), budget / 4 / (m3 * 12)));
+    for (size_t b0 = 0; b0 < bx.size(); b0 += (size_t)batch) {
+      const int nb = (int)std::min<int64_t>(batch, (int64_t)(bx.size() - b0));
+      ppboxes.ensure(std::max<size_t>(ppboxes.n, (size_t)nb));
+      CUBE_HIP(hipMemcpyAsync(ppboxes.p, bx.data() + b0, sizeof(PPBox) * nb, hipMemcpyHostToDevice, s));
+      PPScratch S;
+      S.count = ppcount.ensure((size_t)(nb * m3));
+      S.start = ppstart.ensure((size_t)(nb * m3 + 1));
+      S.scan_tmp = scan_scratch(nb * m3);
+      S.scan_tmp_bytes = scan_tmp.n;
+      int64_t total;
+      {
+        KScope k(&tm, "pp_bin_count", 0);
+        total = pp_bin_count((int)g.izipx, L, ppboxes.p, nb, xpc(), rhocc(), off.p, S, s);
+      }
+      S.entries = ppent.ensure((size_t)std::max<int64_t>(total, 1));
+      S.capacity = total;
+      {
+        KScope k(&tm, "pp_bin_fill", (double)total * 16);
+        pp_bin_fill((int)g.izipx, L, ppboxes.p, nb, xpc(), rhocc(), off.p, S, s);
+      }
+      {
+        KScope k(&tm, "pp_force", (double)total * 16);
+        pp_force(L, ppboxes.p, nb, vpc(), S, o, s);
+      }
+    }
+  }
+}
+
+template <typename R>
+void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* density_host) {
+  if (layout != 1) throw Error(CUBE_ERR_STATE, "kick needs the buffered (extended) particle layout");
+  std::memset(&out, 0, sizeof(out));
+  ensure_offsets();
+  const int mask = in.phase_mask ? in.phase_mask : 15;
+  zero_async(pmout.p, sizeof(PMOut) * 4, s);
+  zero_async(ppout.p, sizeof(PPOut), s);
+  auto fc = [&](int l) -> const FFTCube<R>& {
+    if constexpr (std::is_same<R, float>::value) return fcf[l];
+    else return fcd[l];
+  };
+  const int64_t nnt = g.nnt;
+  // PM1 (kick.f90:34-38, particle_mesh.f90)
+  if (mask & 1) {
+    PMLevel L{};
+    L.n = (int)g.nc; L.m0 = 1; L.periodic = 1; L.win_lo = 1; L.win_hi = (int)g.nt; L.nloop = 1; L.w = 1;
+    L.rcs = (int)g.rcs; L.gridsize = (double)(float)g.rcs; L.nforce = (int)g.nt;
+    L.nte = (int)g.nte; L.ncb = (int)g.ncb; L.nt = (int)g.nt;
+    std::vector<PMBox> bx;
+    for (int64_t T = 0; T < g.ntiles; ++T) {
+      PMBox b{};
+      b.T = (int)T;
+      const int64_t it[3] = {T % nnt, (T / nnt) % nnt, T / (nnt * nnt)};
+      for (int d = 0; d < 3; ++d) {
+        b.nc1[d] = 1; b.nc2[d] = (int)g.nt; b.ushift[d] = 0; b.org[d] = (int)(it[d] * g.nt);
+      }
+      b.moff = 0;
+      bx.push_back(b);
+    }
+    run_level<R>(1, bx, tf[1].p, fc(1), L, in, pmout.p + 0, "pm1_", density_host);
+  }
+  // PM2 (kick.f90:47-69)
+  if (mask & 2) {
+    PMLevel L{};
+    L.n = (int)g.ngt; L.m0 = 1 - (int)g.ngb; L.periodic = 0; L.win_lo = 1 - (int)g.ngb;
+    L.win_hi = (int)(g.ngp + g.ngb); L.nloop = (int)g.ncb; L.w = (int)g.rcs; L.rcs = (int)g.rcs; L.gridsize = 1.0;
+    L.nforce = (int)g.ngp; L.nte = (int)g.nte; L.ncb = (int)g.ncb; L.nt = (int)g.nt;
+    std::vector<PMBox> bx;
+    for (int64_t T = 0; T < g.ntiles; ++T) {
+      PMBox b{};
+      b.T = (int)T;
+      for (int d = 0; d < 3; ++d) { b.nc1[d] = 1; b.nc2[d] = (int)g.nt; b.ushift[d] = 0; b.org[d] = 0; }
+      bx.push_back(b);
+    }
+    run_level<R>(2, bx, tf[2].p, fc(2), L, in, pmout.p + 1, "pm2_");
+  }
+  // classification (kick.f90:108-138)
+  std::vector<int32_t> ires_h(g.ns3, 2);
+  if (mask & 12) {
+    classify_subtiles(g, rhocc(), ires.p, napm.p, s);
+    CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
+    CUBE_HIP(hipMemcpyAsync(out.napm, napm.p, sizeof(int32_t) * 7, hipMemcpyDeviceToHost, s));
+    CUBE_HIP(hipStreamSynchronize(s));
+  }
+  // PM3 (kick.f90:203-239)
+  if ((mask & 4) && cfg.pm3) {
+    const int64_t nns = g.nns;
+    for (int iapm = 2; iapm <= 5; ++iapm) {
+      PMLevel L{};
+      L.n = (int)g.nft[iapm]; L.m0 = 1 - (int)g.nfb[iapm]; L.periodic = 0; L.win_lo = 1 - (int)g.nfb[iapm];
+      L.win_hi = (int)(g.nfp[iapm] + g.nfb[iapm]); L.nloop = 1; L.w = (int)(g.rcs * g.sf[iapm]);
+      L.rcs = (int)g.rcs; L.gridsize = (double)(1.0f / (float)g.sf[iapm]); L.nforce = (int)g.nfp[iapm];
+      L.nte = (int)g.nte; L.ncb = (int)g.ncb; L.nt = (int)g.nt;
+      std::vector<PMBox> bx;
+      for (int64_t it = 0; it < g.ns3; ++it) {
+        if (ires_h[it] != iapm) continue;
+        PMBox b{};
+        const int64_t ifs[3] = {it % nns, (it / nns) % nns, (it / (nns * nns)) % nns};
+        b.T = (int)(it / (nns * nns * nns));
+        for (int d = 0; d < 3; ++d) {
+          b.nc1[d] = (int)(ifs[d] * g.ntt + 1);
+          b.nc2[d] = (int)((ifs[d] + 1) * g.ntt);
+          b.ushift[d] = (int)(ifs[d] * g.nfp[iapm]);
+          b.org[d] = 0;
+        }
+        bx.push_back(b);
+      }
+      char tag[32];
+      snprintf(tag, sizeof(tag), "pm3_%d_", iapm);
+      run_level<R>(3, bx, tf[iapm + 1].p, fc(iapm + 1), L, in, pmout.p + 2, tag);
+    }
+  }
+  // PP (kick.f90:280-298)
+  if ((mask & 8) && cfg.pp) pp_phase(ires_h, in, ppout.p);
+  // collect reductions
+  PMOut po[4];
+  PPOut pq;
+  CUBE_HIP(hipMemcpyAsync(po, pmout.p, sizeof(PMOut) * 4, hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipMemcpyAsync(&pq, ppout.p, sizeof(PPOut), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  auto f2 = [&](const PMOut& p) { return std::is_same<R, float>::value ? p.f2max : (float)p.f2max_d; };
+  auto vm = [&](const PMOut& p, int d) { return std::is_same<R, float>::value ? p.vmax[d] : (float)p.vmax_d[d]; };
+  float vmax[3] = {0, 0, 0};
+  if (mask & 1) {
+    out.f2max_pm1 = f2(po[0]);
+    out.dt_pm1 = dt_from_f2(0.5f, out.f2max_pm1, in.a_mid);
+    out.testrho = std::is_same<R, float>::value ? (double)po[0].testrho_fx / 1099511627776.0 : po[0].testrho_d;
+    for (int d = 0; d < 3; ++d) vmax[d] = vm(po[0], d);
+  }
+  if (mask & 2) {
+    out.f2max_pm2 = f2(po[1]);
+    out.dt_pm2 = dt_from_f2(0.5f, out.f2max_pm2, in.a_mid);
+    for (int d = 0; d < 3; ++d) vmax[d] = vm(po[1], d);
+  }
+  if ((mask & 4) && cfg.pm3) {
+    out.f2max_pm3 = f2(po[2]);
+    out.dt_pm3 = dt_from_f2(0.5f, out.f2max_pm3, in.a_mid);
+    for (int d = 0; d < 3; ++d) vmax[d] = vm(po[2], d);
+  }
+  if ((mask & 8) && cfg.pp) {
+    out.f2max_pp = pq.f2max;
+    out.dt_pp = dt_from_f2(0.05f, pq.f2max, in.a_mid);
+    for (int d = 0; d < 3; ++d) vmax[d] = pq.vmax[d];
+    last_pairs = (int64_t)(pq.pairs_pp / 2 + pq.pairs_pg);
+  }
+  for (int d = 0; d < 3; ++d) out.vmax[d] = vmax[d];
+  out.dt_vmax = (0.9f * 14.0f) / std::max({vmax[0], vmax[1], vmax[2]});  // kick.f90:314
+  out.vz_max = vmax[2];                                                     // kick.f90:315
+}
+
+// drift (drift.f90) on the resident extended layout.
+void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
+  if (layout != 1) throw Error(CUBE_ERR_STATE, "drift needs the buffered (extended) particle layout");
+  ensure_offsets();
+  DriftParams P{};
+  P.dt_mid = (in.dt_old + in.dt) / 2.0f;                                            // drift.f90:21
+  P.nlayer = 2 * (int)std::ceil((P.dt_mid * in.vz_max) / (float)g.rcs) + 2;         // drift.f90:26
+  P.nt = (int)g.nt; P.nte = (int)g.nte; P.ncb = (int)g.ncb; P.nnt = (int)g.nnt; P.rcs = (int)g.rcs;
+  P.izipx = (int)g.izipx;
+  out.nlayer = P.nlayer;
+  // tile overflow (drift.f90:85-93): every particle of the extended tile lands
+  // in the tile's rhoce range, so its total is the extended tile count.
+  std::vector<int64_t> toff(g.ntiles + 1);
+  for (int64_t T = 0; T <= g.ntiles; ++T)
+    CUBE_HIP(hipMemcpyAsync(&toff[T], off.p + T * g.ncell_tile, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  float overhead = 0;
+  for (int64_t T = 0; T < g.ntiles; ++T) {
+    int64_t n = toff[T + 1] - toff[T];
+    overhead = std::max(overhead, (float)n / (float)g.np_tile_max);
+    if (n > g.np_tile_max)
+      throw Error(CUBE_ERR_TILE_OVERFLOW, "too many particles in this tile+buffer: " + std::to_string(n) + " > " +
+                                              std::to_string(g.np_tile_max) + "; please set tile_buffer larger");
+  }
+  out.overhead_tile = overhead;
+  const int64_t nt3 = g.nt * g.nt * g.nt, nphys_cells = g.ntiles * nt3;
+  DriftScratch S;
+  S.flags = dflags.p;
+  S.newcount = dnewcount.ensure(nphys_cells);
+  S.base = dbase.ensure(nphys_cells + 1);
+  zero_async(S.flags, sizeof(int32_t) * 8, s);
+  zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
+  {
+    KScope k(&tm, "drift_shift", (double)np * (3 * esz() + 12));
+    drift_shift(P, xpc(), vpc(), rhocc(), off.p, g.ncells_ext, S, s);
+  }
+  int32_t fl[4];
+  CUBE_HIP(hipMemcpyAsync(fl, S.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
+  {
+    KScope k(&tm, "drift_scan", (double)nphys_cells * 12);
+    scan(S.newcount, S.base, nphys_cells);
+  }
+  int64_t nplocal = 0;
+  CUBE_HIP(hipMemcpyAsync(&nplocal, S.base + nphys_cells, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  if (fl[0] & DRIFT_OUT_OF_RANGE)
+    throw Error(CUBE_ERR_OUT_OF_RANGE, "error: particle out of range (drift beyond 2*ncb coarse cells)");
+  for (int d = 0; d < 3; ++d) {
+    if (fl[1 + d] > kMaxShift) throw Error(CUBE_ERR_OUT_OF_RANGE, "drift shift beyond 16 coarse cells");
+    P.M[d] = fl[1 + d];
+    P.D[d] = 2 * P.M[d] + 1;
+  }
+  ensure_set(1 - cur, nplocal);
+  const int64_t Dt = (int64_t)P.D[0] * P.D[1] * P.D[2];
+  const int64_t per_tile = g.ncell_tile * Dt;
+  const int64_t tbudget = std::max<int64_t>(per_tile, ((int64_t)2 << 30) / 4);
+  const int tg = (int)std::max<int64_t>(1, std::min<int64_t>(g.ntiles, tbudget / per_tile));
+  S.table = dtable.ensure((size_t)(tg * per_tile));
+  S.rank = drank.ensure((size_t)std::max<int64_t>(np, 1));
+  const int other = 1 - cur;
+  for (int t0 = 0; t0 < (int)g.ntiles; t0 += tg) {
+    const int t1 = std::min<int>((int)g.ntiles, t0 + tg);
+    zero_async(S.table, sizeof(int32_t) * (size_t)(t1 - t0) * per_tile, s);
+    {
+      KScope k(&tm, "drift_rank", (double)np * (3 * esz() + 16) * (t1 - t0) / g.ntiles);
+      drift_rank(P, xpc(), vpc(), rhocc(), off.p, t0, t1, S, s);
+    }
+    {
+      KScope k(&tm, "drift_offsets", (double)(t1 - t0) * nt3 * Dt * 8);
+      drift_offsets(P, t0, t1, S, s);
+    }
+    {
+      KScope k(&tm, "drift_scatter", (double)np * 2 * (3 * esz() + 20) * (t1 - t0) / g.ntiles);
+      drift_scatter(P, xpc(), vpc(), pidc(), rhocc(), off.p, t0, t1, S, xp[other].p, vp[other].p,
+                    has_pid ? pid[other].p : nullptr, s);
+    }
+  }
+  int32_t* rnew = rhoc[1 - rcur].p;
+  drift_write_rhoc(P, S, rnew, s);
+  CUBE_HIP(hipMemcpyAsync(fl, S.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  if (fl[0] & 4) throw Error(CUBE_ERR_COUNT_MISMATCH, "warning: density different (rholocal /= rhoce)");
+  if (fl[0] & DRIFT_PID_ZERO) throw Error(CUBE_ERR_PID, "found pid=0");
+  cur = other;
+  rcur = 1 - rcur;
+  layout = 2;
+  np = nplocal;
+  off_valid = false;
+  out.nplocal = nplocal;
+}
+
+// buffer_grid + buffer_x + buffer_v at nn=1.
+void cube_p3m_ctx::buffer(int64_t* np_ext) {
+  if (layout != 2) throw Error(CUBE_ERR_STATE, "buffer needs the physical layout produced by drift");
+  const int64_t nt3 = g.nt * g.nt * g.nt, nphys_cells = g.ntiles * nt3;
+  int32_t* rp = rhocc();
+  int32_t* re = rhoc[1 - rcur].p;
+  physcount.ensure(nphys_cells);
+  physoff.ensure(nphys_cells + 1);
+  {
+    KScope k(&tm, "buffer_rhoc", (double)g.ncells_ext * 8);
+    buffer_rhoc(g, rp, re, physcount.p, s);
+  }
+  rcur = 1 - rcur;
+  scan(re, off.p, g.ncells_ext);
+  scan(physcount.p, physoff.p, nphys_cells);
+  int64_t n = 0;
+  CUBE_HIP(hipMemcpyAsync(&n, off.p + g.ncells_ext, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  if (n > g.np_image_max)
+    throw Error(CUBE_ERR_IMAGE_OVERFLOW, "too many particles in this image+buffer: " + std::to_string(n) + " > " +
+                                             std::to_string(g.np_image_max) + "; please set image_buffer larger");
+  ensure_set(1 - cur, n);
+  const int other = 1 - cur;
+  {
+    KScope k(&tm, "buffer_copy", (double)n * (3 * esz() + 20) * 2);
+    buffer_copy(g, re, off.p, physoff.p, xpc(), vpc(), pidc(), xp[other].p, vp[other].p,
+                has_pid ? pid[other].p : nullptr, s);
+  }
+  cur = other;
+  layout = 1;
+  np = n;
+  off_valid = true;
+  if (np_ext) *np_ext = n;
+}
+
+// ---------------------------------------------------------------------------
+// host spine (spine_image, variables.f90:225-295)
+static void host_spine(const Geo& g, const int32_t* rhoc, int64_t* idx_b_l, int64_t* idx_b_r, int64_t* ppl0,
+                       int64_t* pplr, int64_t* pprl, int64_t* ppr0, int64_t* ppl, int64_t* ppr) {
+  const int64_t nt = g.nt, ncb = g.ncb, nte = g.nte;
+  std::vector<int64_t> bl(g.ntiles * nte * nte), br(g.ntiles * nte * nte);
+  int64_t nsum = 0;
+  for (int64_t T = 0; T < g.ntiles; ++T)
+    for (int64_t k = 1 - ncb; k <= nt + ncb; ++k)
+      for (int64_t j = 1 - ncb; j <= nt + ncb; ++j) {
+        const int64_t r = (T * nte + (k - 1 + ncb)) * nte + (j - 1 + ncb);
+        bl[r] = nsum;
+        for (int64_t i = 1 - ncb; i <= nt + ncb; ++i) nsum += rhoc[rc_index(i, j, k, T, nte, ncb)];
+        br[r] = nsum;
+      }
+  if (idx_b_l) std::memcpy(idx_b_l, bl.data(), sizeof(int64_t) * bl.size());
+  if (idx_b_r) std::memcpy(idx_b_r, br.data(), sizeof(int64_t) * br.size());
+  int64_t nsum_p = 0;
+  for (int64_t T = 0; T < g.ntiles; ++T)
+    for (int64_t k = 1; k <= nt; ++k)
+      for (int64_t j = 1; j <= nt; ++j) {
+        const int64_t q = (T * nt + (k - 1)) * nt + (j - 1);
+        const int64_t r = (T * nte + (k - 1 + ncb)) * nte + (j - 1 + ncb);
+        int64_t np_phy = 0, lsum = 0, rsum = 0, rl = 0, lr = 0;
+        for (int64_t i = 1; i <= nt; ++i) np_phy += rhoc[rc_index(i, j, k, T, nte, ncb)];
+        for (int64_t i = 1 - ncb; i <= 0; ++i) lsum += rhoc[rc_index(i, j, k, T, nte, ncb)];
+        for (int64_t i = nt + 1; i <= nt + ncb; ++i) rsum += rhoc[rc_index(i, j, k, T, nte, ncb)];
+        for (int64_t i = nt - ncb + 1; i <= nt; ++i) rl += rhoc[rc_index(i, j, k, T, nte, ncb)];
+        for (int64_t i = 1; i <= ncb; ++i) lr += rhoc[rc_index(i, j, k, T, nte, ncb)];
+        if (ppl) ppl[q] = nsum_p;
+        nsum_p += np_phy;
+        if (ppr) ppr[q] = nsum_p;
+        if (ppr0) ppr0[q] = br[r] - rsum;
+        if (pprl) pprl[q] = br[r] - rsum - rl;
+        if (ppl0) ppl0[q] = bl[r] + lsum;
+        if (pplr) pplr[q] = bl[r] + lsum + lr;
+      }
+}
+
+static int64_t host_sum(const int32_t* a, int64_t n) {
+  int64_t s = 0;
+  for (int64_t i = 0; i < n; ++i) s += a[i];
+  return s;
+}
+
+template <class F>
+static int guarded(cube_p3m_ctx* ctx, F&& f) {
+  if (!ctx) return CUBE_ERR_ARG;
+  try {
+    CUBE_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return CUBE_OK;
+  } catch (const Error& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return CUBE_ERR_HIP;
+  }
+}
+
+static void upload_ext(cube_p3m_ctx* c, const void* xp, const float* vp, const int64_t* pid, const int32_t* rhoc,
+                       int64_t np_arr) {
+  const int64_t n = host_sum(rhoc, c->g.ncells_ext);
+  if (n > np_arr) throw Error(CUBE_ERR_ARG, "particle arrays shorter than sum(rhoc)");
+  c->ensure_set(0, std::max<int64_t>(n, 1));
+  c->cur = 0;
+  c->rcur = 0;
+  CUBE_HIP(hipMemcpyAsync(c->rhoc[0].p, rhoc, sizeof(int32_t) * c->g.ncells_ext, hipMemcpyHostToDevice, c->s));
+  CUBE_HIP(hipMemcpyAsync(c->xp[0].p, xp, (size_t)n * 3 * c->esz(), hipMemcpyHostToDevice, c->s));
+  CUBE_HIP(hipMemcpyAsync(c->vp[0].p, vp, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->s));
+  c->has_pid = pid != nullptr;
+  if (pid) CUBE_HIP(hipMemcpyAsync(c->pid[0].p, pid, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->s));
+  c->layout = 1;
+  c->np = n;
+  c->off_valid = false;
+}
+
+extern "C" {
+
+const char* cube_p3m_version(void) { return "cube_p3m 0.1 (gfx950)"; }
+
+int cube_p3m_create(const cube_p3m_config* cfg, cube_p3m_ctx** out) {
+  if (!cfg || !out) return CUBE_ERR_ARG;
+  *out = nullptr;
+  auto* c = new cube_p3m_ctx();
+  c->cfg = *cfg;
+  try {
+    c->init();
+  } catch (const Error& e) {
+    int code = e.code;
+    delete c;
+    return code;
+  } catch (...) {
+    delete c;
+    return CUBE_ERR_HIP;
+  }
+  *out = c;
+  return CUBE_OK;
+}
+
+int cube_p3m_destroy(cube_p3m_ctx* ctx) {
+  if (!ctx) return CUBE_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->s);
+  delete ctx;
+  return CUBE_OK;
+}
+
+const char* cube_p3m_last_error(const cube_p3m_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cube_p3m_set_tf(cube_p3m_ctx* ctx, int32_t level, const float* tf, int64_t n) {
+  return guarded(ctx, [&] {
+    if (level < 1 || level > 6 || !tf) throw Error(CUBE_ERR_ARG, "bad tf level");
+    const int64_t m = ctx->tfn[level];
+    if (n != (m / 2 + 1) * m * m) throw Error(CUBE_ERR_ARG, "tf size mismatch");
+    CUBE_HIP(hipMemcpyAsync(ctx->tf[level].p, tf, sizeof(float) * n, hipMemcpyHostToDevice, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int cube_p3m_get_tf(cube_p3m_ctx* ctx, int32_t level, float* tf, int64_t n) {
+  return guarded(ctx, [&] {
+    if (level < 1 || level > 6 || !tf) throw Error(CUBE_ERR_ARG, "bad tf level");
+    const int64_t m = ctx->tfn[level];
+    if (n != (m / 2 + 1) * m * m) throw Error(CUBE_ERR_ARG, "tf size mismatch");
+    CUBE_HIP(hipMemcpyAsync(tf, ctx->tf[level].p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+static void check_spine(cube_p3m_ctx* c, const int32_t* rhoc, const int64_t* idx_b_r) {
+  if (!idx_b_r) return;
+  const Geo& g = c->g;
+  std::vector<int64_t> br(g.ntiles * g.nte * g.nte);
+  host_spine(g, rhoc, nullptr, br.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  for (size_t i = 0; i < br.size(); ++i)
+    if (br[i] != idx_b_r[i]) throw Error(CUBE_ERR_SPINE, "idx_b_r does not match the cumulative sum of rhoc");
+}
+
+int cube_p3m_particle_mesh(cube_p3m_ctx* ctx, const void* xp, float* vp, const int32_t* rhoc,
+                           const int64_t* idx_b_r, int64_t np, const cube_kick_in* in, float* density,
+                           cube_kick_out* out) {
+  return guarded(ctx, [&] {
+    if (!xp || !vp || !rhoc || !in) throw Error(CUBE_ERR_ARG, "null argument");
+    check_spine(ctx, rhoc, idx_b_r);
+    upload_ext(ctx, xp, vp, nullptr, rhoc, np);
+    cube_kick_in k = *in;
+    k.phase_mask = 1;
+    cube_kick_out o;
+    if (ctx->prec == 32) ctx->kick<float>(k, o, density);
+    else ctx->kick<double>(k, o, density);
+    CUBE_HIP(hipMemcpyAsync(vp, ctx->vpc(), sizeof(float) * 3 * ctx->np, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+    if (out) *out = o;
+  });
+}
+
+int cube_p3m_kick(cube_p3m_ctx* ctx, const void* xp, float* vp, const int32_t* rhoc, const int64_t* idx_b_r,
+                  int64_t np, const cube_kick_in* in, cube_kick_out* out) {
+  return guarded(ctx, [&] {
+    if (!xp || !vp || !rhoc || !in) throw Error(CUBE_ERR_ARG, "null argument");
+    check_spine(ctx, rhoc, idx_b_r);
+    upload_ext(ctx, xp, vp, nullptr, rhoc, np);
+    cube_kick_out o;
+    if (ctx->prec == 32) ctx->kick<float>(*in, o, nullptr);
+    else ctx->kick<double>(*in, o, nullptr);
+    CUBE_HIP(hipMemcpyAsync(vp, ctx->vpc(), sizeof(float) * 3 * ctx->np, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+    if (out) *out = o;
+  });
+}
+
+int cube_p3m_drift(cube_p3m_ctx* ctx, void* xp, float* vp, int64_t* pid, int32_t* rhoc, int64_t np,
+                   int64_t* idx_b_l, int64_t* idx_b_r, int64_t* ppl0, int64_t* pplr, int64_t* pprl, int64_t* ppr0,
+                   int64_t* ppl, int64_t* ppr, const cube_drift_in* in, cube_drift_out* out) {
+  return guarded(ctx, [&] {
+    if (!xp || !vp || !rhoc || !in) throw Error(CUBE_ERR_ARG, "null argument");
+    upload_ext(ctx, xp, vp, pid, rhoc, np);
+    cube_drift_out o{};
+    ctx->drift(*in, o);
+    const int64_t n = o.nplocal;
+    if (n > np) throw Error(CUBE_ERR_IMAGE_OVERFLOW, "output arrays too short for nplocal");
+    CUBE_HIP(hipMemcpyAsync(xp, ctx->xpc(), (size_t)n * 3 * ctx->esz(), hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(vp, ctx->vpc(), sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->s));
+    if (pid) CUBE_HIP(hipMemcpyAsync(pid, ctx->pidc(), sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(rhoc, ctx->rhocc(), sizeof(int32_t) * ctx->g.ncells_ext, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+    // xp(:,nplocal+1:)=0 etc. (drift.f90:183-186)
+    std::memset((char*)xp + (size_t)n * 3 * ctx->esz(), 0, (size_t)(np - n) * 3 * ctx->esz());
+    std::memset(vp + 3 * n, 0, sizeof(float) * 3 * (np - n));
+    if (pid) std::memset(pid + n, 0, sizeof(int64_t) * (np - n));
+    host_spine(ctx->g, rhoc, idx_b_l, idx_b_r, ppl0, pplr, pprl, ppr0, ppl, ppr);
+    if (out) *out = o;
+  });
+}
+
+int cube_p3m_buffer(cube_p3m_ctx* ctx, void* xp, float* vp, int64_t* pid, int32_t* rhoc, int64_t nplocal,
+                    int64_t np_cap, int64_t* idx_b_l, int64_t* idx_b_r, int64_t* np_ext) {
+  return guarded(ctx, [&] {
+    if (!xp || !vp || !rhoc) throw Error(CUBE_ERR_ARG, "null argument");
+    const Geo& g = ctx->g;
+    // physical counts only: buffers are ignored (filled here)
+    std::vector<int32_t> rp(rhoc, rhoc + g.ncells_ext);
+    int64_t nphys = 0;
+    for (int64_t T = 0; T < g.ntiles; ++T)
+      for (int64_t k = 1 - g.ncb; k <= g.nt + g.ncb; ++k)
+        for (int64_t j = 1 - g.ncb; j <= g.nt + g.ncb; ++j)
+          for (int64_t i = 1 - g.ncb; i <= g.nt + g.ncb; ++i) {
+            const int64_t c = rc_index(i, j, k, T, g.nte, g.ncb);
+            const bool ph = i >= 1 && i <= g.nt && j >= 1 && j <= g.nt && k >= 1 && k <= g.nt;
+            if (!ph) rp[c] = 0;
+            else nphys += rp[c];
+          }
+    if (nphys != nplocal) throw Error(CUBE_ERR_ARG, "nplocal differs from the physical sum of rhoc");
+    ctx->ensure_set(0, std::max<int64_t>(nplocal, 1));
+    ctx->cur = 0;
+    ctx->rcur = 0;
+    CUBE_HIP(hipMemcpyAsync(ctx->rhoc[0].p, rp.data(), sizeof(int32_t) * g.ncells_ext, hipMemcpyHostToDevice, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(ctx->xp[0].p, xp, (size_t)nplocal * 3 * ctx->esz(), hipMemcpyHostToDevice, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(ctx->vp[0].p, vp, sizeof(float) * 3 * nplocal, hipMemcpyHostToDevice, ctx->s));
+    ctx->has_pid = pid != nullptr;
+    if (pid) CUBE_HIP(hipMemcpyAsync(ctx->pid[0].p, pid, sizeof(int64_t) * nplocal, hipMemcpyHostToDevice, ctx->s));
+    ctx->layout = 2;
+    ctx->np = nplocal;
+    int64_t n = 0;
+    ctx->buffer(&n);
+    if (n > np_cap) throw Error(CUBE_ERR_IMAGE_OVERFLOW, "output arrays too short for the buffered layout");
+    CUBE_HIP(hipMemcpyAsync(xp, ctx->xpc(), (size_t)n * 3 * ctx->esz(), hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(vp, ctx->vpc(), sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->s));
+    if (pid) CUBE_HIP(hipMemcpyAsync(pid, ctx->pidc(), sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(rhoc, ctx->rhocc(), sizeof(int32_t) * g.ncells_ext, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+    host_spine(g, rhoc, idx_b_l, idx_b_r, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (np_ext) *np_ext = n;
+  });
+}
+
+int cube_p3m_state_upload(cube_p3m_ctx* ctx, const void* xp, const float* vp, const int64_t* pid,
+                          const int32_t* rhoc, int64_t np) {
+  return guarded(ctx, [&] {
+    if (!xp || !vp || !rhoc) throw Error(CUBE_ERR_ARG, "null argument");
+    upload_ext(ctx, xp, vp, pid, rhoc, np);
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int cube_p3m_state_download(cube_p3m_ctx* ctx, void* xp, float* vp, int64_t* pid, int32_t* rhoc, int64_t np_cap,
+                            int64_t* np) {
+  return guarded(ctx, [&] {
+    if (ctx->layout == 0) throw Error(CUBE_ERR_STATE, "no resident state");
+    const int64_t n = ctx->np;
+    if (np) *np = n;
+    if (n > np_cap) throw Error(CUBE_ERR_ARG, "download capacity too small");
+    if (xp) CUBE_HIP(hipMemcpyAsync(xp, ctx->xpc(), (size_t)n * 3 * ctx->esz(), hipMemcpyDeviceToHost, ctx->s));
+    if (vp) CUBE_HIP(hipMemcpyAsync(vp, ctx->vpc(), sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->s));
+    if (pid && ctx->has_pid)
+      CUBE_HIP(hipMemcpyAsync(pid, ctx->pidc(), sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx->s));
+    if (rhoc)
+      CUBE_HIP(hipMemcpyAsync(rhoc, ctx->rhocc(), sizeof(int32_t) * ctx->g.ncells_ext, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int cube_p3m_kick_resident(cube_p3m_ctx* ctx, const cube_kick_in* in, cube_kick_out* out) {
+  return guarded(ctx, [&] {
+    if (!in) throw Error(CUBE_ERR_ARG, "null argument");
+    cube_kick_out o;
+    if (ctx->prec == 32) ctx->kick<float>(*in, o, nullptr);
+    else ctx->kick<double>(*in, o, nullptr);
+    if (out) *out = o;
+  });
+}
+
+int cube_p3m_drift_resident(cube_p3m_ctx* ctx, const cube_drift_in* in, cube_drift_out* out) {
+  return guarded(ctx, [&] {
+    if (!in) throw Error(CUBE_ERR_ARG, "null argument");
+    cube_drift_out o{};
+    ctx->drift(*in, o);
+    if (out) *out = o;
+  });
+}
+
+int cube_p3m_buffer_resident(cube_p3m_ctx* ctx, int64_t* np_ext) {
+  return guarded(ctx, [&] { ctx->buffer(np_ext); });
+}
+
+int cube_p3m_step_resident(cube_p3m_ctx* ctx, const cube_drift_in* din, const cube_kick_in* kin,
+                           cube_drift_out* dout, cube_kick_out* kout) {
+  return guarded(ctx, [&] {
+    if (!din || !kin) throw Error(CUBE_ERR_ARG, "null argument");
+    cube_drift_out d{};
+    ctx->drift(*din, d);
+    int64_t n = 0;
+    ctx->buffer(&n);
+    cube_kick_out k;
+    if (ctx->prec == 32) ctx->kick<float>(*kin, k, nullptr);
+    else ctx->kick<double>(*kin, k, nullptr);
+    if (dout) *dout = d;
+    if (kout) *kout = k;
+  });
+}
+
+int cube_p3m_make_ic(cube_p3m_ctx* ctx, const cube_ic_params* ic, float* vz_max, int64_t* nplocal) {
+  return guarded(ctx, [&] {
+    if (!ic) throw Error(CUBE_ERR_ARG, "null argument");
+    const Geo& g = ctx->g;
+    ctx->ensure_set(ctx->cur, std::max<int64_t>(g.np_image, 1));
+    ctx->has_pid = true;
+    ICResult r = make_ic(g, *ic, ctx->xp[ctx->cur].p, ctx->vp[ctx->cur].p, ctx->pid[ctx->cur].p,
+                         ctx->rhoc[ctx->rcur].p, ctx->s);
+    ctx->layout = 2;
+    ctx->np = r.nplocal;
+    ctx->off_valid = false;
+    int64_t n = 0;
+    ctx->buffer(&n);
+    if (vz_max) *vz_max = r.vz_max;
+    if (nplocal) *nplocal = r.nplocal;
+  });
+}
+
+int cube_p3m_synchronize(cube_p3m_ctx* ctx) {
+  return guarded(ctx, [&] { CUBE_HIP(hipStreamSynchronize(ctx->s)); });
+}
+
+int cube_p3m_timing_enable(cube_p3m_ctx* ctx, int32_t enable) {
+  return guarded(ctx, [&] { ctx->tm.enabled = enable != 0; });
+}
+
+int cube_p3m_timing_reset(cube_p3m_ctx* ctx) {
+  return guarded(ctx, [&] { ctx->tm.reset(); });
+}
+
+int cube_p3m_timing_report(cube_p3m_ctx* ctx, char* buf, int64_t len) {
+  return guarded(ctx, [&] {
+    std::string r = ctx->tm.report();
+    char extra[128];
+    snprintf(extra, sizeof(extra), "pp_pairs_last %lld 0 0 0\n", (long long)ctx->last_pairs);
+    r += extra;
+    if (buf && len > 0) {
+      std::strncpy(buf, r.c_str(), (size_t)len - 1);
+      buf[len - 1] = 0;
+    }
+  });
+}
+
+}  // extern "C"

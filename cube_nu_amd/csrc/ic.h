@@ -1,0 +1,24 @@
+// ic.h -- seeded Zel'dovich particle generator (the algorithm of
+// utilities/ic.f90:231-796, with a counter-based RNG and global binning).
+#pragma once
+#include "../../include/cube_p3m.h"
+#include "common.h"
+
+namespace cube {
+
+struct ICResult {
+  int64_t nplocal;
+  float vz_max;
+};
+
+// Writes the physical layout (tile order, cells (k,j,i), ascending pid inside
+// a cell) into xp/vp/pid (capacity >= ng^3) and rhoc (extended layout, zero
+// buffers).
+ICResult make_ic(const Geo& g, const cube_ic_params& p, void* xp, float* vp, int64_t* pid, int32_t* rhoc,
+                 hipStream_t s);
+
+// Host helpers exposed for tests of the cosmology inputs.
+double bbks_sigma8_norm(const cube_ic_params& p);
+double dgrow(double a, double om, double ol);
+
+}  // namespace cube

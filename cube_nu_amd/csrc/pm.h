@@ -1,0 +1,62 @@
+// pm.h -- particle-mesh kernels shared by PM1 (coarse), PM2 (tile) and PM3
+// (sub-tile) levels: TSC deposit, fused gradient + interpolation + kick,
+// Green's function generation and sub-tile classification.
+#pragma once
+#include "common.h"
+
+namespace cube {
+
+// Per-level geometry (the type_pm record of variables.f90:94-113).
+struct PMLevel {
+  int n;             // mesh points per dim of one box array
+  int64_t bstride;   // elements between consecutive boxes in the mesh array
+  int m0;            // Fortran mesh index held at array index 0 (box-local + org)
+  int periodic;      // 1 for the coarse mesh (buffer_potential wrap)
+  int win_lo, win_hi;// deposit window (box-local Fortran mesh indices)
+  int nloop;         // coarse ring read around the box's cells (pm%nloop)
+  int w;             // fine mesh cells per coarse cell (integer, for bounds)
+  int rcs;           // ratio_cs
+  double gridsize;   // pm%gridsize as stored (real(4) promoted)
+  int nforce;        // pm%nforce: physical force cells 1..nforce
+  int nte, ncb, nt;  // tile geometry for rhoc indexing
+};
+
+// One box: a tile (PM1, PM2) or a sub-tile (PM3).
+struct PMBox {
+  int T;             // 0-based tile index
+  int nc1[3], nc2[3];// physical coarse cells (tile-local Fortran indices)
+  int ushift[3];     // utile_shift * nfp(iapm)
+  int org[3];        // global coarse offset (PM1), else 0
+  int64_t moff;      // element offset of this box's mesh array
+};
+
+struct PMOut {  // device-side reductions of one level
+  float f2max;
+  float vmax[3];
+  double f2max_d;
+  double vmax_d[3];
+  unsigned long long testrho_fx;  // PM1 density sum in 2^-40 fixed point
+  double testrho_d;
+};
+
+template <typename R>
+void pm_deposit(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp, const int32_t* rhoc,
+                const int64_t* cell_off, R* mesh, float mass, PMOut* out, hipStream_t s);
+
+template <typename R>
+void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp, float* vp,
+               const int32_t* rhoc, const int64_t* cell_off, const R* phi, float a_mid, float dt, PMOut* out,
+               hipStream_t s);
+
+// Green_3D (Green/Green.f90:1-80) evaluated in FP64 on the device, stored as
+// FP32 (n/2+1, n, n) and scaled by `factor` ((1-f_nu) or 1).
+void green_device(int n, float apm_soft, float apm_range, float gridsize, double factor, float* out, hipStream_t s);
+
+// Sub-tile classification (kick.f90:110-133): ires(4096) and napm counts.
+void classify_subtiles(const Geo& g, const int32_t* rhoc, int32_t* ires, int32_t* napm, hipStream_t s);
+
+// Exclusive scan of int32 counts into int64 offsets (n+1 outputs).
+void scan_counts(const int32_t* in, int64_t* out, int64_t n, void* tmp, size_t tmp_bytes, hipStream_t s);
+size_t scan_tmp_bytes(int64_t n);
+
+}  // namespace cube

@@ -1,0 +1,46 @@
+// pp.h -- short-range particle-particle force (PP_Force, kick.f90:376-588).
+#pragma once
+#include "common.h"
+
+namespace cube {
+
+struct PPLevel {
+  int sf;          // ratio_sf(iapm) = rcp
+  int npgrid;      // ntt * rcp
+  int ntt, nte, ncb, rcs;
+  float pp_range, pp_range2, app;
+  float mass_scale, v_scale;
+};
+
+struct PPBox {
+  int T;
+  int ifs[3];      // 0-based sub-tile coordinates within the tile
+};
+
+struct PPOut {
+  float f2max;
+  float vmax[3];
+  unsigned long long pairs_pp;  // directed physical-physical pair count
+  unsigned long long pairs_pg;  // physical-ghost pair count
+};
+
+// Scratch the caller provides for one level batch.
+struct PPScratch {
+  int32_t* count = nullptr;   // nbox * m^3
+  int64_t* start = nullptr;   // nbox * m^3 + 1
+  float4* entries = nullptr;  // capacity entries
+  int64_t capacity = 0;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+
+// Bin the particles of each sub-tile (plus its ghost ring) into its PP grid.
+// Returns the number of entries needed (host sync); fills when it fits.
+int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nbox, const void* xp,
+                     const int32_t* rhoc, const int64_t* cell_off, PPScratch& S, hipStream_t s);
+void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nbox, const void* xp, const int32_t* rhoc,
+                 const int64_t* cell_off, PPScratch& S, hipStream_t s);
+void pp_force(const PPLevel& L, const PPBox* boxes_dev, int nbox, float* vp, PPScratch& S, PPOut* out,
+              hipStream_t s);
+
+}  // namespace cube

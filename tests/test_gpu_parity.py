@@ -1,0 +1,134 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on
+identical seeded inputs (ng=64: 262k particles, 64 tiles of 4^3 coarse cells,
+4096 sub-tiles).
+
+Tolerances (north star): rms relative error of the velocity change (i.e. of
+the force) <= 1e-5 per phase; integer outputs (napm, drift order, counts,
+re-encoded coordinates, PID permutation) bit-exact.
+"""
+import numpy as np
+import pytest
+
+from cube_nu_amd import P3M
+from cube_nu_amd.p3m import CubeError, CUBE_ERR_OUT_OF_RANGE
+from inputs import buffered_set, oparams, physical_set, rel_rms
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+NG = 64
+A_MID, DT = 0.5, 0.3
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    p = P3M(NG, izipx=2, precision=32)
+    yield p
+    p.close()
+
+
+@pytest.fixture(scope="module")
+def data():
+    # one dense coarse cell (2000 particles) puts its sub-tile at ires=3
+    dense = [((5, 9, 2), 2000)]
+    return buffered_set(_geo(), seed=11, mean=6.0, dense=dense)
+
+
+def _geo():
+    from cube_nu_amd.p3m import Geometry
+
+    return Geometry(NG)
+
+
+@pytest.fixture(scope="module")
+def tfs():
+    return orc.tf_default(oparams(_geo()))
+
+
+def test_green_matches_oracle(gpu, tfs):
+    for level in range(1, 7):
+        g = gpu.get_tf(level)
+        o = tfs[level - 1]
+        scale = np.abs(o).max()
+        assert np.max(np.abs(g - o)) <= 2e-6 * scale, level
+
+
+@pytest.mark.parametrize("mask", [1, 2, 4, 8])
+def test_phase_parity(gpu, data, tfs, mask):
+    xp, vp, pid, rhoc = data
+    v_gpu = vp.copy()
+    v_orc = vp.copy()
+    out_g = gpu.kick(xp, v_gpu, rhoc, a_mid=A_MID, dt=DT, phase_mask=mask)
+    out_o, _, _ = orc.kick(oparams(_geo()), xp, v_orc, rhoc, tfs, A_MID, DT, phase_mask=mask)
+    dv_g = v_gpu - vp
+    dv_o = v_orc - vp
+    err = rel_rms(dv_g, dv_o)
+    print(f"mask {mask}: rms rel dv err {err:.3e}")
+    assert err <= 1e-5
+    key = {1: "f2max_pm1", 2: "f2max_pm2", 4: "f2max_pm3", 8: "f2max_pp"}[mask]
+    assert abs(out_g[key] - out_o[key]) <= 1e-5 * abs(out_o[key])
+    if mask & 12:
+        assert out_g["napm"] == out_o["napm"]
+
+
+def test_pm1_density_and_testrho(gpu, data, tfs):
+    xp, vp, pid, rhoc = data
+    v = vp.copy()
+    dens, out = gpu.particle_mesh(xp, v, rhoc, a_mid=A_MID, dt=DT)
+    v2 = vp.copy()
+    out_o, rho1, _ = orc.kick(oparams(_geo()), xp, v2, rhoc, tfs, A_MID, DT, phase_mask=1, want_rho1=True)
+    assert rel_rms(dens, rho1) <= 1e-6
+    assert abs(out["testrho"] - out_o["testrho"]) <= 1e-6 * out_o["testrho"]
+
+
+def test_full_kick(gpu, data, tfs):
+    xp, vp, pid, rhoc = data
+    v_gpu = vp.copy()
+    v_orc = vp.copy()
+    out_g = gpu.kick(xp, v_gpu, rhoc, a_mid=A_MID, dt=DT)
+    out_o, _, _ = orc.kick(oparams(_geo()), xp, v_orc, rhoc, tfs, A_MID, DT)
+    assert rel_rms(v_gpu - vp, v_orc - vp) <= 1e-5
+    for k in ("dt_pm1", "dt_pm2", "dt_pm3", "dt_pp", "dt_vmax", "vz_max"):
+        assert abs(out_g[k] - out_o[k]) <= 1e-5 * abs(out_o[k]), k
+
+
+def test_drift_bit_exact(gpu, data):
+    xp, vp, pid, rhoc = data
+    vz = float(np.abs(vp[:, 2]).max())
+    st, xo, vo, po, ro, oo = orc.drift(oparams(_geo()), xp, vp, pid, rhoc, 0.4, 0.6, vz)
+    assert st == 0
+    xg, vg, pg, rg = xp.copy(), vp.copy(), pid.copy(), rhoc.copy()
+    og = gpu.drift(xg, vg, pg, rg, dt_old=0.4, dt=0.6, vz_max=vz)
+    n = og["nplocal"]
+    assert n == oo["nplocal"] and og["nlayer"] == oo["nlayer"]
+    assert np.array_equal(rg, ro)
+    assert np.array_equal(xg[:n], xo)
+    assert np.array_equal(vg[:n], vo)
+    assert np.array_equal(pg[:n], po)
+    assert not xg[n:].any() and not pg[n:].any()
+
+
+def test_buffer_bit_exact(gpu):
+    geo = _geo()
+    xp, vp, pid, rhoc = physical_set(geo, seed=5, mean=5.0)
+    n = xp.shape[0]
+    cap = int(n * (geo.nte / geo.nt) ** 3 * 1.2) + 1024
+    xe, ve, pe, re = orc.buffer(oparams(geo), xp, vp, pid, rhoc, cap)
+    xg = np.zeros((cap, 3), np.int16)
+    vg = np.zeros((cap, 3), np.float32)
+    pg = np.zeros(cap, np.int64)
+    xg[:n], vg[:n], pg[:n] = xp, vp, pid
+    rg = rhoc.copy()
+    m = gpu.buffer(xg, vg, pg, rg, n)
+    assert m == xe.shape[0]
+    assert np.array_equal(rg, re)
+    assert np.array_equal(xg[:m], xe) and np.array_equal(vg[:m], ve) and np.array_equal(pg[:m], pe)
+
+
+def test_drift_out_of_range_error(gpu, data):
+    xp, vp, pid, rhoc = data
+    v = vp.copy()
+    v[5, 0] = 1e6  # moves far beyond 2*ncb coarse cells (drift.f90:64-73)
+    with pytest.raises(CubeError) as e:
+        gpu.drift(xp.copy(), v, pid.copy(), rhoc.copy(), dt_old=0.5, dt=0.5, vz_max=1.0)
+    assert e.value.code == CUBE_ERR_OUT_OF_RANGE

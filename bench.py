@@ -74,8 +74,9 @@ def roofline(rep, steps):
     avg_ms = v["ms"] / v["calls"]
     phases = {k: round(vv["ms"] / steps, 3) for k, vv in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
     if name == "pp_force":
+        # 40 FP32 flops per interacting pair, each unordered pair once
         pairs = rep.get("pp_pairs_last", {}).get("calls", 0)
-        ach = 40.0 * pairs / (avg_ms * 1e-3) / 1e12 / max(1, v["calls"] // steps)
+        ach = 40.0 * pairs / (v["ms"] / steps * 1e-3) / 1e12
         r = {"kernel": name, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
              "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "avg_launch_ms": avg_ms}
     else:

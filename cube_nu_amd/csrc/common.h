@@ -130,11 +130,16 @@ __device__ inline double kick_update(double vp, double dv, float a_mid, float dt
 }
 
 // F_ra_inline (kick.f90:360-374) with integer powers expanded as products.
+// c must equal (1./35.)/apm**2 evaluated in R (f_ra_coef), passed in so that
+// kernels compute it once.
 template <typename R>
-__host__ __device__ inline R f_ra(R r, R apm) {
+__host__ __device__ inline R f_ra_coef(R apm) {
+  return ((R)1 / (R)35) / (apm * apm);
+}
+template <typename R>
+__host__ __device__ inline R f_ra(R r, R apm, R c) {
   R ep = ((R)2 * r) / apm;
   if (apm == (R)0 || ep > (R)2) return (R)1 / (r * r);
-  R c = ((R)1 / (R)35) / (apm * apm);
   R e2 = ep * ep, e3 = e2 * ep, e4 = e2 * e2, e5 = e4 * ep, e6 = e3 * e3;
   if (ep > (R)1)
     return c * (((((((R)12 * ((R)1 / e2) - (R)224) + (R)896 * ep) - (R)840 * e2) + (R)224 * e3) + (R)70 * e4 -
@@ -145,13 +150,24 @@ __host__ __device__ inline R f_ra(R r, R apm) {
 }
 
 // Order-preserving float max for non-negative values through integer atomics
-// (deterministic regardless of launch order).
+// (deterministic regardless of launch order).  The atomic is issued only when
+// the value exceeds what the location already holds: a running maximum
+// saturates after a few blocks, so almost every block only reads, instead of
+// serialising on one address.
 __device__ inline void atomic_max_nonneg(float* addr, float v) {
-  atomicMax((int*)addr, __float_as_int(v));
+  if (v > *(volatile float*)addr) atomicMax((int*)addr, __float_as_int(v));
 }
 __device__ inline void atomic_max_nonneg(double* addr, double v) {
-  atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+  if (v > *(volatile double*)addr) atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
+__device__ inline void atomic_max_int(int* addr, int v) {
+  if (v > *(volatile int*)addr) atomicMax(addr, v);
+}
+
+// Integer sums spread over kSumSlots addresses (selected by block index) so
+// that block-level atomics do not serialise on one address; the host adds
+// the slots.
+constexpr int kSumSlots = 64;
 
 template <typename R>
 struct Cplx;

@@ -77,14 +77,27 @@ __global__ void k_drift_shift(DriftParams P, const XT<IZ>* __restrict__ xp, cons
   const int64_t base = off[c];
   const int lo = 1 - 2 * P.ncb, hi = P.nt + 2 * P.ncb;
   int err = 0, ms[3] = {0, 0, 0};
-  for (int l = lane; l < cnt; l += 64) {
-    int g[3];
-    dest_cell<IZ>(P, xp, vp, base + l, ijk, g);
-    for (int d = 0; d < 3; ++d)
-      if (g[d] < lo || g[d] > hi) err = 1;
-    if (physical(P, g)) {
-      for (int d = 0; d < 3; ++d) ms[d] = max(ms[d], abs(g[d] - ijk[d]));
-      atomicAdd(&newcount[pcell(P, T, g)], 1);
+  for (int l0 = 0; l0 < cnt; l0 += 64) {
+    const int l = l0 + lane;
+    int64_t key = -1;
+    if (l < cnt) {
+      int g[3];
+      dest_cell<IZ>(P, xp, vp, base + l, ijk, g);
+      for (int d = 0; d < 3; ++d)
+        if (g[d] < lo || g[d] > hi) err = 1;
+      if (physical(P, g)) {
+        for (int d = 0; d < 3; ++d) ms[d] = max(ms[d], abs(g[d] - ijk[d]));
+        key = pcell(P, T, g);
+      }
+    }
+    // one integer atomic per distinct destination in the wavefront
+    unsigned long long active = __ballot(key >= 0);
+    while (active) {
+      const int leader = __ffsll((long long)active) - 1;
+      const int64_t lk = __shfl(key, leader, 64);
+      const unsigned long long peers = __ballot(key == lk);
+      if (lane == leader) atomicAdd(&newcount[lk], (int)__popcll(peers));
+      active &= ~peers;
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -93,7 +106,7 @@ __global__ void k_drift_shift(DriftParams P, const XT<IZ>* __restrict__ xp, cons
   }
   if (lane == 0 && cnt > 0) {
     if (err) atomicOr(&flags[0], DRIFT_OUT_OF_RANGE);
-    for (int d = 0; d < 3; ++d) atomicMax(&flags[1 + d], ms[d]);
+    for (int d = 0; d < 3; ++d) atomic_max_int(&flags[1 + d], ms[d]);
   }
 }
 

@@ -99,6 +99,7 @@ struct cube_p3m_ctx {
   DevBuf<int32_t> ppcount;
   DevBuf<int64_t> ppstart;
   DevBuf<float4> ppent;
+  DevBuf<int> ppovf;
   DevBuf<int32_t> dflags, drank, dtable, dnewcount;
   DevBuf<int64_t> dbase;
   DevBuf<int32_t> physcount;
@@ -290,6 +291,8 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
     L.pp_range = 3.5f / (float)L.sf;  // apm3 (parameters.f90:43)
     L.pp_range2 = L.pp_range * L.pp_range;
     L.app = cfg.app;
+    L.c_app = f_ra_coef<float>(L.app);
+    L.c_range = f_ra_coef<float>(L.pp_range);
     L.mass_scale = mass_scale;
     L.v_scale = v_scale;
     const int64_t m = L.npgrid + 2, m3 = m * m * m;
@@ -304,6 +307,7 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
       S.start = ppstart.ensure((size_t)(nb * m3 + 1));
       S.scan_tmp = scan_scratch(nb * m3);
       S.scan_tmp_bytes = scan_tmp.n;
+      S.ovf = ppovf.ensure((size_t)(pp_bricks(L, nb) + 1));
       int64_t total;
       {
         KScope k(&tm, "pp_bin_count", 0);
@@ -420,7 +424,13 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
   if (mask & 1) {
     out.f2max_pm1 = f2(po[0]);
     out.dt_pm1 = dt_from_f2(0.5f, out.f2max_pm1, in.a_mid);
-    out.testrho = std::is_same<R, float>::value ? (double)po[0].testrho_fx / 1099511627776.0 : po[0].testrho_d;
+    unsigned long long tfx = 0;
+    double td = 0;
+    for (int q = 0; q < kSumSlots; ++q) {
+      tfx += po[0].testrho_fx[q];
+      td += po[0].testrho_d[q];
+    }
+    out.testrho = std::is_same<R, float>::value ? (double)tfx / 1099511627776.0 : td;
     for (int d = 0; d < 3; ++d) vmax[d] = vm(po[0], d);
   }
   if (mask & 2) {
@@ -437,7 +447,12 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
     out.f2max_pp = pq.f2max;
     out.dt_pp = dt_from_f2(0.05f, pq.f2max, in.a_mid);
     for (int d = 0; d < 3; ++d) vmax[d] = pq.vmax[d];
-    last_pairs = (int64_t)(pq.pairs_pp / 2 + pq.pairs_pg);
+    unsigned long long npp = 0, npg = 0;
+    for (int q = 0; q < kSumSlots; ++q) {
+      npp += pq.pairs_pp[q];
+      npg += pq.pairs_pg[q];
+    }
+    last_pairs = (int64_t)(npp / 2 + npg);
   }
   for (int d = 0; d < 3; ++d) out.vmax[d] = vmax[d];
   out.dt_vmax = (0.9f * 14.0f) / std::max({vmax[0], vmax[1], vmax[2]});  // kick.f90:314

@@ -36,6 +36,12 @@ template <typename R>
 void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox, R div3,
                   R div1, hipStream_t s, KTimer* tm = nullptr, const char* tag = "");
 
+// Size-specialised 3-pass form (fft_plane.hip); returns false when n has no
+// specialisation.
+template <typename R>
+bool fft_convolve_plane(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox,
+                        R div3, R div1, hipStream_t s, KTimer* tm, const char* tag);
+
 // Forward-only and inverse-only pieces (used by the input maker and tests).
 template <typename R>
 void fft_r2c3d(const FFTCube<R>& c, const R* rho, typename Cplx<R>::T* out, int64_t nbox, hipStream_t s);

@@ -281,6 +281,13 @@ KTimer& null_timer() {
   return t;
 }
 
+// CUBE_FFT_GENERIC=1 forces the generic five-pass line kernels (tests use it
+// to check the specialised path against them).
+bool use_plane_path() {
+  const char* e = std::getenv("CUBE_FFT_GENERIC");
+  return !(e && e[0] == '1');
+}
+
 template <typename R>
 int rows_per_block(int h) {
   int bytes_per_row = 2 * (h + 1) * (int)sizeof(typename Cplx<R>::T);
@@ -370,6 +377,8 @@ template <typename R>
 void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox, R div3,
                   R div1, hipStream_t s, KTimer* tm, const char* tag) {
   using C = typename Cplx<R>::T;
+  if (use_plane_path() && fft_convolve_plane<R>(c, rho, work, G, nbox, div3, div1, s, tm ? tm : &null_timer(), tag))
+    return;
   const int n = c.n, h = n / 2;
   const int64_t nrows = nbox * n * n;
   const double Gb = (double)n * n * n * sizeof(R) * nbox;          // real cube bytes

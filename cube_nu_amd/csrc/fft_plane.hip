@@ -1,0 +1,357 @@
+// fft_plane.hip -- size-specialised 3-pass FFT convolution for cubes with
+// n <= 160: (A) per z-plane, r2c along x and forward c2c along y in LDS;
+// (B) per y-slice, forward c2c along z, times G, inverse along z in LDS;
+// (C) per z-plane, inverse along y and c2r along x, with the normalisation.
+// The radix plan, line counts and per-thread butterfly counts are
+// compile-time constants; stages run in place in LDS (each thread loads its
+// butterflies into registers, barrier, writes them back).
+#include "fft.h"
+#include "timing.h"
+
+namespace cube {
+namespace {
+
+struct Fac {
+  int n;
+  int r[16];
+};
+constexpr Fac factor(int n) {
+  Fac f{};
+  int m = n;
+  while (m % 4 == 0) { f.r[f.n++] = 4; m /= 4; }
+  while (m % 2 == 0) { f.r[f.n++] = 2; m /= 2; }
+  while (m % 3 == 0) { f.r[f.n++] = 3; m /= 3; }
+  while (m % 5 == 0) { f.r[f.n++] = 5; m /= 5; }
+  return f;  // m == 1 for every specialised size
+}
+
+template <typename C>
+struct CT;
+template <>
+struct CT<float2> {
+  using R = float;
+  __device__ static float2 mk(float a, float b) { return make_float2(a, b); }
+};
+template <>
+struct CT<double2> {
+  using R = double;
+  __device__ static double2 mk(double a, double b) { return make_double2(a, b); }
+};
+template <typename C>
+__device__ inline C add(C a, C b) { return CT<C>::mk(a.x + b.x, a.y + b.y); }
+template <typename C>
+__device__ inline C sub(C a, C b) { return CT<C>::mk(a.x - b.x, a.y - b.y); }
+template <typename C>
+__device__ inline C mul(C a, C b) { return CT<C>::mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename C>
+__device__ inline C scl(C a, typename CT<C>::R s) { return CT<C>::mk(a.x * s, a.y * s); }
+template <typename C>
+__device__ inline C miu(C a, typename CT<C>::R u) { return CT<C>::mk(-u * a.y, u * a.x); }  // i*u*a
+template <typename C>
+__device__ inline C cj(C a) { return CT<C>::mk(a.x, -a.y); }
+
+template <int SIGN, int R, typename C>
+__device__ inline void bfly(C* v) {
+  using RR = typename CT<C>::R;
+  const RR s = (RR)SIGN;
+  if constexpr (R == 2) {
+    C a = v[0], b = v[1];
+    v[0] = add(a, b);
+    v[1] = sub(a, b);
+  } else if constexpr (R == 4) {
+    C a = add(v[0], v[2]), b = sub(v[0], v[2]);
+    C c = add(v[1], v[3]), d = miu(sub(v[1], v[3]), s);
+    v[0] = add(a, c);
+    v[2] = sub(a, c);
+    v[1] = add(b, d);
+    v[3] = sub(b, d);
+  } else if constexpr (R == 3) {
+    const RR c = (RR)-0.5, u = s * (RR)0.86602540378443864676;
+    C t1 = add(v[1], v[2]);
+    C m = add(v[0], scl(t1, c));
+    C r = miu(sub(v[1], v[2]), u);
+    v[0] = add(v[0], t1);
+    v[1] = add(m, r);
+    v[2] = sub(m, r);
+  } else {
+    static_assert(R == 5, "radix");
+    const RR c1 = (RR)0.30901699437494742410, c2 = (RR)-0.80901699437494742410;
+    const RR s1 = (RR)0.95105651629515357212, s2 = (RR)0.58778525229247312917;
+    C a1 = add(v[1], v[4]), a2 = add(v[2], v[3]);
+    C b1 = sub(v[1], v[4]), b2 = sub(v[2], v[3]);
+    C p1 = add(v[0], add(scl(a1, c1), scl(a2, c2)));
+    C p2 = add(v[0], add(scl(a1, c2), scl(a2, c1)));
+    C q1 = miu(add(scl(b1, s1), scl(b2, s2)), s);
+    C q2 = miu(sub(scl(b1, s2), scl(b2, s1)), s);
+    v[0] = add(v[0], add(a1, a2));
+    v[1] = add(p1, q1);
+    v[4] = sub(p1, q1);
+    v[2] = add(p2, q2);
+    v[3] = sub(p2, q2);
+  }
+}
+
+// In-place Stockham over NL lines of length N held in LDS; element e of line
+// l lives at buf[l*LS + e*ES].  LFAST selects the thread mapping (lines
+// fastest for strided lines, elements fastest for contiguous lines).  tw is
+// the W_{N*TWS} table; twiddles are W_N^e = tw[e*TWS].
+template <int SIGN, int N, int NL, int LS, int ES, bool LFAST, int NT, int S, int NS, int TWS, typename C>
+__device__ inline void stages(C* buf, const C* __restrict__ tw) {
+  constexpr Fac F = factor(N);
+  if constexpr (S < F.n) {
+    constexpr int R = F.r[S];
+    constexpr int NB = N / R;
+    constexpr int TOT = NL * NB;
+    constexpr int MB = (TOT + NT - 1) / NT;
+    constexpr int STEP = N / (NS * R);
+    C v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int idx = threadIdx.x + b * NT;
+      if (idx < TOT) {
+        const int l = LFAST ? idx % NL : idx / NB;
+        const int j = LFAST ? idx / NL : idx % NB;
+        const int jm = j % NS;
+        const C* sp = buf + l * LS + j * ES;
+        v[b][0] = sp[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C w = tw[(r * jm * STEP) * TWS];
+          if (SIGN > 0) w.y = -w.y;
+          v[b][r] = mul(sp[r * NB * ES], w);
+        }
+        bfly<SIGN, R>(v[b]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int idx = threadIdx.x + b * NT;
+      if (idx < TOT) {
+        const int l = LFAST ? idx % NL : idx / NB;
+        const int j = LFAST ? idx / NL : idx % NB;
+        const int jm = j % NS;
+        C* dp = buf + l * LS + ((j / NS) * NS * R + jm) * ES;
+#pragma unroll
+        for (int k = 0; k < R; ++k) dp[k * NS * ES] = v[b][k];
+      }
+    }
+    __syncthreads();
+    stages<SIGN, N, NL, LS, ES, LFAST, NT, S + 1, NS * R, TWS>(buf, tw);
+  }
+}
+
+template <typename R>
+using CP = typename Cplx<R>::T;
+
+// Pass A: one z-plane: rho (N x N reals) -> complex (N rows of N/2+1).
+template <typename R, int N, int NT>
+__global__ void __launch_bounds__(NT) k_plane_fwd(const R* __restrict__ rho, CP<R>* __restrict__ out,
+                                                  const CP<R>* __restrict__ tw) {
+  using C = CP<R>;
+  constexpr int H = N / 2, P = H + 1;
+  __shared__ C buf[N * P];
+  const int64_t plane = blockIdx.x;  // box * N + z
+  const R* src = rho + plane * N * N;
+  for (int i = threadIdx.x; i < N * H; i += NT) {
+    const int y = i / H, j = i % H;
+    buf[y * P + j] = CT<C>::mk(src[y * N + 2 * j], src[y * N + 2 * j + 1]);
+  }
+  __syncthreads();
+  stages<-1, H, N, P, 1, false, NT, 0, 1, 2>(buf, tw);  // half-length rows
+  // split: X[k] = (Z[k]+conj Z[H-k])/2 - i/2 W_N^k (Z[k]-conj Z[H-k]), k=0..H
+  const R half = (R)0.5;
+  constexpr int NPAIR = H / 2 + 1;
+  C xa[(N * NPAIR + NT - 1) / NT], xb[(N * NPAIR + NT - 1) / NT];
+#pragma unroll
+  for (int b = 0; b < (N * NPAIR + NT - 1) / NT; ++b) {
+    const int i = threadIdx.x + b * NT;
+    if (i < N * NPAIR) {
+      const int y = i / NPAIR, k = i % NPAIR;
+      const int k2 = H - k;
+      C zk = buf[y * P + (k == H ? 0 : k)], zk2 = buf[y * P + (k2 == H ? 0 : k2)];
+      // X[k]
+      {
+        C zc = cj(zk2);
+        C xe = scl(add(zk, zc), half), d = scl(sub(zk, zc), half);
+        C xo = CT<C>::mk(d.y, -d.x);
+        xa[b] = add(xe, mul(tw[k], xo));
+      }
+      // X[H-k]
+      {
+        C zc = cj(zk);
+        C xe = scl(add(zk2, zc), half), d = scl(sub(zk2, zc), half);
+        C xo = CT<C>::mk(d.y, -d.x);
+        xb[b] = add(xe, mul(tw[k2], xo));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < (N * NPAIR + NT - 1) / NT; ++b) {
+    const int i = threadIdx.x + b * NT;
+    if (i < N * NPAIR) {
+      const int y = i / NPAIR, k = i % NPAIR;
+      buf[y * P + k] = xa[b];
+      buf[y * P + (H - k)] = xb[b];
+    }
+  }
+  __syncthreads();
+  stages<-1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);  // columns along y
+  C* dst = out + plane * N * P;
+  for (int i = threadIdx.x; i < N * P; i += NT) dst[i] = buf[i];
+}
+
+// Pass B: one y-slice of one box: lines along z, times G, back.
+template <typename R, int N, int NT>
+__global__ void __launch_bounds__(NT) k_zslice(CP<R>* __restrict__ data, const float* __restrict__ G,
+                                               const CP<R>* __restrict__ tw) {
+  using C = CP<R>;
+  constexpr int P = N / 2 + 1;
+  __shared__ C buf[N * P];
+  const int64_t box = blockIdx.x / N;
+  const int y = blockIdx.x % N;
+  C* base = data + box * (int64_t)N * N * P + (int64_t)y * P;
+  for (int i = threadIdx.x; i < N * P; i += NT) {
+    const int z = i / P, x = i % P;
+    buf[i] = base[(int64_t)z * N * P + x];
+  }
+  __syncthreads();
+  stages<-1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);
+  const float* gb = G + (int64_t)y * P;
+  for (int i = threadIdx.x; i < N * P; i += NT) {
+    const int z = i / P, x = i % P;
+    const R g = (R)gb[(int64_t)z * N * P + x];
+    buf[i] = CT<C>::mk(buf[i].x * g, buf[i].y * g);  // cxyz*tf (kick.f90:698)
+  }
+  __syncthreads();
+  stages<+1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);
+  for (int i = threadIdx.x; i < N * P; i += NT) {
+    const int z = i / P, x = i % P;
+    base[(int64_t)z * N * P + x] = buf[i];
+  }
+}
+
+// Pass C: one z-plane: inverse along y, c2r along x, normalisation.
+template <typename R, int N, int NT>
+__global__ void __launch_bounds__(NT) k_plane_inv(const CP<R>* __restrict__ in, R* __restrict__ phi,
+                                                  const CP<R>* __restrict__ tw, R div3, R div1) {
+  using C = CP<R>;
+  constexpr int H = N / 2, P = H + 1;
+  __shared__ C buf[N * P];
+  const int64_t plane = blockIdx.x;
+  const C* src = in + plane * N * P;
+  for (int i = threadIdx.x; i < N * P; i += NT) buf[i] = src[i];
+  __syncthreads();
+  stages<+1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);
+  // Z[k] = E + iO with E = X[k] + conj X[H-k], O = (X[k] - conj X[H-k]) W_N^-k
+  constexpr int NPAIR = H / 2 + 1;
+  constexpr int MB = (N * NPAIR + NT - 1) / NT;
+  C za[MB], zb[MB];
+#pragma unroll
+  for (int b = 0; b < MB; ++b) {
+    const int i = threadIdx.x + b * NT;
+    if (i < N * NPAIR) {
+      const int y = i / NPAIR, k = i % NPAIR, k2 = H - k;
+      const C xk = buf[y * P + k], xk2 = buf[y * P + k2];
+      {
+        C xc = cj(xk2), w = tw[k];
+        w.y = -w.y;
+        za[b] = add(add(xk, xc), miu(mul(sub(xk, xc), w), (typename CT<C>::R)1));
+      }
+      {
+        C xc = cj(xk), w = tw[k2];
+        w.y = -w.y;
+        zb[b] = add(add(xk2, xc), miu(mul(sub(xk2, xc), w), (typename CT<C>::R)1));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MB; ++b) {
+    const int i = threadIdx.x + b * NT;
+    if (i < N * NPAIR) {
+      const int y = i / NPAIR, k = i % NPAIR, k2 = H - k;
+      if (k < H) buf[y * P + k] = za[b];
+      if (k2 < H) buf[y * P + k2] = zb[b];
+    }
+  }
+  __syncthreads();
+  stages<+1, H, N, P, 1, false, NT, 0, 1, 2>(buf, tw);
+  R* dst = phi + plane * N * N;
+  for (int i = threadIdx.x; i < N * H; i += NT) {
+    const int y = i / H, j = i % H;
+    const C z = buf[y * P + j];
+    R a = z.x, b = z.y;
+    if (div3 != (R)0) {
+      a = a / div3;
+      b = b / div3;
+    } else {
+      a = ((a / div1) / div1) / div1;
+      b = ((b / div1) / div1) / div1;
+    }
+    dst[y * N + 2 * j] = a;
+    dst[y * N + 2 * j + 1] = b;
+  }
+}
+
+template <typename R, int N>
+bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t nbox, R div3, R div1,
+               hipStream_t s, KTimer* tm, const std::string& t) {
+  // large planes: more threads so each holds fewer butterflies in registers
+  constexpr int NT = N >= 128 ? 1024 : 256;
+  const double Gb = (double)N * N * N * sizeof(R) * nbox;
+  const double Kb = (double)N * N * (N / 2 + 1) * sizeof(CP<R>) * nbox;
+  const double Gk = (double)N * N * (N / 2 + 1) * sizeof(float);
+  {
+    KScope k(tm, (t + "fft_xy_fwd").c_str(), Gb + Kb);
+    hipLaunchKernelGGL((k_plane_fwd<R, N, NT>), dim3((unsigned)(nbox * N)), dim3(NT), 0, s, rho, work, c.tw);
+    CUBE_HIP(hipGetLastError());
+  }
+  {
+    KScope k(tm, (t + "fft_z_green").c_str(), 2 * Kb + Gk);
+    hipLaunchKernelGGL((k_zslice<R, N, NT>), dim3((unsigned)(nbox * N)), dim3(NT), 0, s, work, G, c.tw);
+    CUBE_HIP(hipGetLastError());
+  }
+  {
+    KScope k(tm, (t + "fft_xy_inv").c_str(), Gb + Kb);
+    hipLaunchKernelGGL((k_plane_inv<R, N, NT>), dim3((unsigned)(nbox * N)), dim3(NT), 0, s, work, rho, c.tw, div3,
+                       div1);
+    CUBE_HIP(hipGetLastError());
+  }
+  return true;
+}
+
+}  // namespace
+
+// Returns false when n has no specialised kernel (the caller then uses the
+// generic line passes).
+template <typename R>
+bool fft_convolve_plane(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox,
+                        R div3, R div1, hipStream_t s, KTimer* tm, const char* tag) {
+  std::string t(tag);
+  switch (c.n) {
+    case 16: return run_plane<R, 16>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 24: return run_plane<R, 24>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 32: return run_plane<R, 32>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 48: return run_plane<R, 48>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 64: return run_plane<R, 64>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 72: return run_plane<R, 72>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 80: return run_plane<R, 80>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 96: return run_plane<R, 96>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    default: break;
+  }
+  // n^2 (n/2+1) planes of FP64 beyond 96 do not fit in LDS
+  if constexpr (std::is_same<R, float>::value) {
+    if (c.n == 128) return run_plane<R, 128>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    if (c.n == 160) return run_plane<R, 160>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+  }
+  return false;
+}
+
+template bool fft_convolve_plane<float>(const FFTCube<float>&, float*, float2*, const float*, int64_t, float, float,
+                                        hipStream_t, KTimer*, const char*);
+template bool fft_convolve_plane<double>(const FFTCube<double>&, double*, double2*, const float*, int64_t, double,
+                                         double, hipStream_t, KTimer*, const char*);
+
+}  // namespace cube

@@ -35,8 +35,8 @@ struct PMOut {  // device-side reductions of one level
   float vmax[3];
   double f2max_d;
   double vmax_d[3];
-  unsigned long long testrho_fx;  // PM1 density sum in 2^-40 fixed point
-  double testrho_d;
+  unsigned long long testrho_fx[kSumSlots];  // PM1 density sum in 2^-40 fixed point
+  double testrho_d[kSumSlots];
 };
 
 template <typename R>

@@ -64,7 +64,48 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1), ncz = max(0, chi[2] - clo[2] + 1);
   const int nC = ncx * ncy * ncz;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int c = wave; c < nC; c += nw) {
+  if (L.w == 1) {
+    // Coarse mesh: every particle of cell (ci,cj,ck) has floor(xpos) = ci-1
+    // etc., so all of them hit the same 27 mesh cells.  Lanes keep 27 partial
+    // sums, the wavefront reduces them in a fixed tree, and one lane adds
+    // each to LDS.
+    for (int c = wave; c < nC; c += nw) {
+      const int ci = clo[0] + c % ncx, cj = clo[1] + (c / ncx) % ncy, ck = clo[2] + c / (ncx * ncy);
+      const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
+      const int cnt = rhoc[cell];
+      if (cnt == 0) continue;
+      const int64_t base = off[cell];
+      const int ijk[3] = {ci, cj, ck};
+      Acc a27[27];
+#pragma unroll
+      for (int q = 0; q < 27; ++q) a27[q] = 0;
+      for (int l = lane; l < cnt; l += 64) {
+        R w[3][3];
+        for (int d = 0; d < 3; ++d) {
+          double xpos = ((double)(ijk[d] - 1) + XP<IZ>::off(xp, base + l, d)) * (double)L.rcs / L.gridsize -
+                        (double)B.ushift[d];
+          tsc_weights<R>(xpos, w[d]);
+        }
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+          const R cw = (w[0][q % 3] * w[1][(q / 3) % 3]) * w[2][q / 9];
+          if constexpr (std::is_same<R, float>::value) a27[q] += (Acc)__double2ull_rn((double)cw * kFix);
+          else a27[q] += cw;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 27; ++q) {
+        Acc v = a27[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        // taps floor(xpos) + {0,1,2} = ci-1, ci, ci+1 (particle_mesh.f90:56-58)
+        const int mx = ci - 1 + q % 3, my = cj - 1 + (q / 3) % 3, mz = ck - 1 + q / 9;
+        if (lane == 0 && mx >= mlo[0] && mx <= mhi[0] && my >= mlo[1] && my <= mhi[1] && mz >= mlo[2] &&
+            mz <= mhi[2])
+          atomicAdd(&acc[((mz - mlo[2]) * BD + (my - mlo[1])) * BD + (mx - mlo[0])], v);
+      }
+    }
+  }
+  for (int c = wave; c < (L.w == 1 ? 0 : nC); c += nw) {
     const int ci = clo[0] + c % ncx, cj = clo[1] + (c / ncx) % ncy, ck = clo[2] + c / (ncx * ncy);
     const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
     const int cnt = rhoc[cell];
@@ -122,8 +163,8 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
     if (threadIdx.x == 0) {
       Acc s = 0;
       for (int q = 0; q < nw; ++q) s += red[q];
-      if constexpr (std::is_same<R, float>::value) atomicAdd(&out->testrho_fx, s);
-      else atomicAdd(&out->testrho_d, s);
+      if constexpr (std::is_same<R, float>::value) atomicAdd(&out->testrho_fx[blockIdx.x % kSumSlots], s);
+      else atomicAdd(&out->testrho_d[blockIdx.x % kSumSlots], s);
     }
   }
 }
@@ -419,10 +460,12 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
-template <typename R, int IZ>
-void launch_deposit(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
-                    const int64_t* off, R* mesh, float mass, PMOut* out, int testrho, hipStream_t s) {
-  constexpr int BD = std::is_same<R, float>::value ? 16 : 8;
+// Block edge: 16 fine cells for the tile/sub-tile levels (FP32), 8 for the
+// coarse level (one mesh cell per coarse cell: 8^3 cells already hold ~32k
+// particles) and for the FP64 form (LDS size).
+template <typename R, int IZ, int BD>
+void launch_deposit_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
+                       const int64_t* off, R* mesh, float mass, PMOut* out, int testrho, hipStream_t s) {
   const int nblk = (L.win_hi - L.win_lo + 1 + BD - 1) / BD;
   const int64_t grid = (int64_t)nblk * nblk * nblk * nbox;
   hipLaunchKernelGGL((k_deposit<R, IZ, BD>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
@@ -431,14 +474,31 @@ void launch_deposit(const PMLevel& L, const PMBox* boxes, int nbox, const void* 
 }
 
 template <typename R, int IZ>
-void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
-                   const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
-  constexpr int BD = std::is_same<R, float>::value ? 16 : 8;
+void launch_deposit(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
+                    const int64_t* off, R* mesh, float mass, PMOut* out, int testrho, hipStream_t s) {
+  if (std::is_same<R, float>::value && L.w > 1)
+    launch_deposit_bd<R, IZ, 16>(L, boxes, nbox, xp, rhoc, off, mesh, mass, out, testrho, s);
+  else
+    launch_deposit_bd<R, IZ, 8>(L, boxes, nbox, xp, rhoc, off, mesh, mass, out, testrho, s);
+}
+
+template <typename R, int IZ, int BD>
+void launch_interp_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
+                      const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
   const int nblk = (L.nforce + BD - 1) / BD;
   const int64_t grid = (int64_t)nblk * nblk * nblk * nbox;
   hipLaunchKernelGGL((k_interp<R, IZ, BD>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
                      (const XT<IZ>*)xp, vp, rhoc, off, phi, a_mid, dt, out);
   CUBE_HIP(hipGetLastError());
+}
+
+template <typename R, int IZ>
+void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
+                   const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
+  if (std::is_same<R, float>::value && L.w > 1)
+    launch_interp_bd<R, IZ, 16>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
+  else
+    launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
 }
 
 }  // namespace

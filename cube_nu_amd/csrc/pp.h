@@ -9,6 +9,7 @@ struct PPLevel {
   int npgrid;      // ntt * rcp
   int ntt, nte, ncb, rcs;
   float pp_range, pp_range2, app;
+  float c_app, c_range;  // (1./35.)/apm**2 for app and pp_range
   float mass_scale, v_scale;
 };
 
@@ -20,8 +21,8 @@ struct PPBox {
 struct PPOut {
   float f2max;
   float vmax[3];
-  unsigned long long pairs_pp;  // directed physical-physical pair count
-  unsigned long long pairs_pg;  // physical-ghost pair count
+  unsigned long long pairs_pp[kSumSlots];  // directed physical-physical pair count
+  unsigned long long pairs_pg[kSumSlots];  // physical-ghost pair count
 };
 
 // Scratch the caller provides for one level batch.
@@ -32,6 +33,7 @@ struct PPScratch {
   int64_t capacity = 0;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
+  int* ovf = nullptr;          // [0] count, [1..] brick ids for the global path
 };
 
 // Bin the particles of each sub-tile (plus its ghost ring) into its PP grid.
@@ -42,5 +44,7 @@ void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nbox, 
                  const int64_t* cell_off, PPScratch& S, hipStream_t s);
 void pp_force(const PPLevel& L, const PPBox* boxes_dev, int nbox, float* vp, PPScratch& S, PPOut* out,
               hipStream_t s);
+// Number of force bricks of a batch (size of the overflow list).
+int64_t pp_bricks(const PPLevel& L, int nbox);
 
 }  // namespace cube

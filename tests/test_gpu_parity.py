@@ -132,3 +132,18 @@ def test_drift_out_of_range_error(gpu, data):
     with pytest.raises(CubeError) as e:
         gpu.drift(xp.copy(), v, pid.copy(), rhoc.copy(), dt_old=0.5, dt=0.5, vz_max=1.0)
     assert e.value.code == CUBE_ERR_OUT_OF_RANGE
+
+
+@pytest.mark.parametrize("mask", [1, 2, 4])
+def test_fft_plane_path_matches_generic(data, mask, monkeypatch):
+    """The size-specialised 3-pass FFT convolution agrees with the generic
+    five-pass line kernels (both FP32) on the same deposit."""
+    xp, vp, pid, rhoc = data
+    gpu = P3M(NG)
+    v1 = vp.copy()
+    gpu.kick(xp, v1, rhoc, a_mid=A_MID, dt=DT, phase_mask=mask)
+    monkeypatch.setenv("CUBE_FFT_GENERIC", "1")
+    v2 = vp.copy()
+    gpu.kick(xp, v2, rhoc, a_mid=A_MID, dt=DT, phase_mask=mask)
+    gpu.close()
+    assert rel_rms(v1 - vp, v2 - vp) <= 2e-6

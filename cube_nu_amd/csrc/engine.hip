@@ -99,7 +99,6 @@ struct cube_p3m_ctx {
   DevBuf<int32_t> ppcount;
   DevBuf<int64_t> ppstart;
   DevBuf<float4> ppent;
-  DevBuf<int> ppovf;
   DevBuf<int32_t> dflags, drank, dtable, dnewcount;
   DevBuf<int64_t> dbase;
   DevBuf<int32_t> physcount;
@@ -293,6 +292,8 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
     L.app = cfg.app;
     L.c_app = f_ra_coef<float>(L.app);
     L.c_range = f_ra_coef<float>(L.pp_range);
+    L.inv_app = 1.0f / L.app;
+    L.inv_range = 1.0f / L.pp_range;
     L.mass_scale = mass_scale;
     L.v_scale = v_scale;
     const int64_t m = L.npgrid + 2, m3 = m * m * m;
@@ -307,7 +308,6 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
       S.start = ppstart.ensure((size_t)(nb * m3 + 1));
       S.scan_tmp = scan_scratch(nb * m3);
       S.scan_tmp_bytes = scan_tmp.n;
-      S.ovf = ppovf.ensure((size_t)(pp_bricks(L, nb) + 1));
       int64_t total;
       {
         KScope k(&tm, "pp_bin_count", 0);
@@ -321,7 +321,7 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
       }
       {
         KScope k(&tm, "pp_force", (double)total * 16);
-        pp_force(L, ppboxes.p, nb, vpc(), S, o, s);
+        pp_force(L, prec, nb, vpc(), S, o, s);
       }
     }
   }

@@ -33,6 +33,70 @@ __device__ inline int floor_div(int a, int b) {
 template <int IZ>
 using XT = typename XP<IZ>::T;
 
+constexpr int kMaxCells = 1000;  // coarse cells one block reads: (8+2)^3 on the coarse level
+
+// Exclusive block scan of cnt[0..n) into pre[0..n] (pre[n] = total), n <= 1024.
+__device__ inline void block_scan(const int* cnt, int* pre, int n, int* wsum) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b0 = 4 * threadIdx.x;
+  int loc[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    loc[k] = (b0 + k < n) ? cnt[b0 + k] : 0;
+    s += loc[k];
+  }
+  int incl = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wave; ++w) wbase += wsum[w];
+  int run = wbase + incl - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (b0 + k < n) pre[b0 + k] = run;
+    run += loc[k];
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    int tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += wsum[w];
+    pre[n] = tot;
+  }
+  __syncthreads();
+}
+
+// Counts and offsets of the coarse cells [clo, chi]^3 of a box, staged in
+// LDS with one round of independent loads.  Returns the number of cells.
+__device__ inline int stage_cells(const PMLevel& L, const PMBox& B, const int clo[3], const int chi[3],
+                                  const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off, int* cnt,
+                                  int* pre, int64_t* gof, int* wsum) {
+  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1), ncz = max(0, chi[2] - clo[2] + 1);
+  const int nC = ncx * ncy * ncz;
+  for (int t = threadIdx.x; t < nC; t += blockDim.x) {
+    const int ci = clo[0] + t % ncx, cj = clo[1] + (t / ncx) % ncy, ck = clo[2] + t / (ncx * ncy);
+    const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
+    cnt[t] = rhoc[cell];
+    gof[t] = off[cell];
+  }
+  __syncthreads();
+  block_scan(cnt, pre, nC, wsum);
+  return nC;
+}
+
+// largest c with pre[c] <= i
+__device__ inline int find_cell(const int* pre, int nC, int i) {
+  int lo = 0, hi = nC - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <typename R, int IZ, int BD>
 __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restrict__ boxes,
                                                  const XT<IZ>* __restrict__ xp, const int32_t* __restrict__ rhoc,
@@ -41,6 +105,8 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
   __shared__ Acc acc[BD * BD * BD];
   __shared__ Acc red[256 / 64];
+  __shared__ int cnt[kMaxCells], pre[kMaxCells + 1], wsum[4];
+  __shared__ int64_t gof[kMaxCells];
   const int nblk = (L.win_hi - L.win_lo + 1 + BD - 1) / BD;
   int64_t b = blockIdx.x;
   const int bx = (int)(b % nblk);
@@ -60,9 +126,8 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
     chi[d] = min(floor_div(mhi[d] + B.ushift[d], L.w) + 1, B.nc2[d] + L.nloop);
   }
   for (int t = threadIdx.x; t < BD * BD * BD; t += blockDim.x) acc[t] = 0;
-  __syncthreads();
-  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1), ncz = max(0, chi[2] - clo[2] + 1);
-  const int nC = ncx * ncy * ncz;
+  const int nC = stage_cells(L, B, clo, chi, rhoc, off, cnt, pre, gof, wsum);
+  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   if (L.w == 1) {
     // Coarse mesh: every particle of cell (ci,cj,ck) has floor(xpos) = ci-1
@@ -70,16 +135,15 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
     // sums, the wavefront reduces them in a fixed tree, and one lane adds
     // each to LDS.
     for (int c = wave; c < nC; c += nw) {
+      const int n = cnt[c];
+      if (n == 0) continue;
       const int ci = clo[0] + c % ncx, cj = clo[1] + (c / ncx) % ncy, ck = clo[2] + c / (ncx * ncy);
-      const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
-      const int cnt = rhoc[cell];
-      if (cnt == 0) continue;
-      const int64_t base = off[cell];
+      const int64_t base = gof[c];
       const int ijk[3] = {ci, cj, ck};
       Acc a27[27];
 #pragma unroll
       for (int q = 0; q < 27; ++q) a27[q] = 0;
-      for (int l = lane; l < cnt; l += 64) {
+      for (int l = lane; l < n; l += 64) {
         R w[3][3];
         for (int d = 0; d < 3; ++d) {
           double xpos = ((double)(ijk[d] - 1) + XP<IZ>::off(xp, base + l, d)) * (double)L.rcs / L.gridsize -
@@ -104,37 +168,53 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
           atomicAdd(&acc[((mz - mlo[2]) * BD + (my - mlo[1])) * BD + (mx - mlo[0])], v);
       }
     }
-  }
-  for (int c = wave; c < (L.w == 1 ? 0 : nC); c += nw) {
-    const int ci = clo[0] + c % ncx, cj = clo[1] + (c / ncx) % ncy, ck = clo[2] + c / (ncx * ncy);
-    const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
-    const int cnt = rhoc[cell];
-    const int64_t base = off[cell];
-    const int ijk[3] = {ci, cj, ck};
-    for (int l = lane; l < cnt; l += 64) {
-      const int64_t ip = base + l;
-      R w[3][3];
-      int f[3];
-      for (int d = 0; d < 3; ++d) {
-        double xpos = ((double)(ijk[d] - 1) + XP<IZ>::off(xp, ip, d)) * (double)L.rcs / L.gridsize - (double)B.ushift[d];
-        f[d] = tsc_weights<R>(xpos, w[d]);
+  } else {
+    // Fine meshes: particles of all the block's cells in one flat loop,
+    // four per thread in flight.
+    const int total = pre[nC];
+    for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
+      int ijk[4][3];
+      int64_t ip[4];
+      double xo[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * 256;
+        ip[u] = -1;
+        if (i < total) {
+          const int c = find_cell(pre, nC, i);
+          ijk[u][0] = clo[0] + c % ncx;
+          ijk[u][1] = clo[1] + (c / ncx) % ncy;
+          ijk[u][2] = clo[2] + c / (ncx * ncy);
+          ip[u] = gof[c] + (i - pre[c]);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) xo[u][d] = XP<IZ>::off(xp, ip[u], d);
+        }
       }
-      for (int i3 = 0; i3 < 3; ++i3) {
-        const int mz = f[2] + i3;
-        if (mz < mlo[2] || mz > mhi[2]) continue;
-        for (int i2 = 0; i2 < 3; ++i2) {
-          const int my = f[1] + i2;
-          if (my < mlo[1] || my > mhi[1]) continue;
-          const R wyz = w[1][i2];
-          for (int i1 = 0; i1 < 3; ++i1) {
-            const int mx = f[0] + i1;
-            if (mx < mlo[0] || mx > mhi[0]) continue;
-            const R cw = (w[0][i1] * wyz) * w[2][i3];  // dx(1,i1)*dx(2,i2)*dx(3,i3)
-            const int li = ((mz - mlo[2]) * BD + (my - mlo[1])) * BD + (mx - mlo[0]);
-            if constexpr (std::is_same<R, float>::value) {
-              atomicAdd(&acc[li], (unsigned long long)__double2ull_rn((double)cw * kFix));
-            } else {
-              atomicAdd(&acc[li], cw);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (ip[u] < 0) continue;
+        R w[3][3];
+        int f[3];
+        for (int d = 0; d < 3; ++d) {
+          double xpos = ((double)(ijk[u][d] - 1) + xo[u][d]) * (double)L.rcs / L.gridsize - (double)B.ushift[d];
+          f[d] = tsc_weights<R>(xpos, w[d]);
+        }
+        for (int i3 = 0; i3 < 3; ++i3) {
+          const int mz = f[2] + i3;
+          if (mz < mlo[2] || mz > mhi[2]) continue;
+          for (int i2 = 0; i2 < 3; ++i2) {
+            const int my = f[1] + i2;
+            if (my < mlo[1] || my > mhi[1]) continue;
+            const R wyz = w[1][i2];
+            for (int i1 = 0; i1 < 3; ++i1) {
+              const int mx = f[0] + i1;
+              if (mx < mlo[0] || mx > mhi[0]) continue;
+              const R cw = (w[0][i1] * wyz) * w[2][i3];  // dx(1,i1)*dx(2,i2)*dx(3,i3)
+              const int li = ((mz - mlo[2]) * BD + (my - mlo[1])) * BD + (mx - mlo[0]);
+              if constexpr (std::is_same<R, float>::value)
+                atomicAdd(&acc[li], (unsigned long long)__double2ull_rn((double)cw * kFix));
+              else
+                atomicAdd(&acc[li], cw);
             }
           }
         }
@@ -189,8 +269,11 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
   constexpr int FS = BD + 2, PS = BD + 6;
   constexpr int NF = (FS * FS * FS + 255) / 256;
   constexpr int LDSN = (PS * PS * PS > 3 * FS * FS * FS) ? PS * PS * PS : 3 * FS * FS * FS;
+  constexpr int MAXC = kMaxCells;
   __shared__ R lds[LDSN];
   __shared__ R red[4];
+  __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4];
+  __shared__ int64_t gof[MAXC];
   const int nblk = (L.nforce + BD - 1) / BD;
   int64_t b = blockIdx.x;
   const int bx = (int)(b % nblk);
@@ -201,7 +284,14 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
   const int ibox = (int)(b / nblk);
   const PMBox B = boxes[ibox];
   const int f0[3] = {bx * BD, by * BD, bz * BD};
-  // 1. phi(f0-2 .. f0+BD+3) into LDS
+  // coarse cells whose particles can have floor(xpos) in this block; a cell i
+  // covers floor in [w(i-1)-s, wi-s-1]
+  int clo[3], chi[3];
+  for (int d = 0; d < 3; ++d) {
+    clo[d] = max(B.nc1[d], floor_div(f0[d] + B.ushift[d], L.w) + 1);
+    chi[d] = min(B.nc2[d], floor_div(f0[d] + BD - 1 + B.ushift[d], L.w) + 1);
+  }
+  // 1. phi(f0-2 .. f0+BD+3) into LDS, cell metadata alongside
   for (int t = threadIdx.x; t < PS * PS * PS; t += blockDim.x) {
     const int l[3] = {t % PS, (t / PS) % PS, t / (PS * PS)};
     int a[3];
@@ -213,7 +303,8 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
     }
     lds[t] = ok ? phi[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]] : (R)0;
   }
-  __syncthreads();
+  const int nC = stage_cells(L, B, clo, chi, rhoc, off, cnt, pre, gof, wsum);  // contains barriers
+  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1);
   // 2. force(d, f0 .. f0+BD+1) = sum(phi(m + [-2,-1,1,2]) * weight)
   const FDWeights<R> W;
   R fr[NF][3];
@@ -260,30 +351,35 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
     }
   }
   __syncthreads();
-  // 3. particles whose floor(xpos) lies in this block
-  int clo[3], chi[3];
-  for (int d = 0; d < 3; ++d) {
-    // a coarse cell i covers floor(xpos) in [w(i-1)-s, wi-s-1]
-    clo[d] = max(B.nc1[d], floor_div(f0[d] + B.ushift[d], L.w) + 1);
-    chi[d] = min(B.nc2[d], floor_div(f0[d] + BD - 1 + B.ushift[d], L.w) + 1);
-  }
-  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1), ncz = max(0, chi[2] - clo[2] + 1);
-  const int nC = ncx * ncy * ncz;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  // 3. particles whose floor(xpos) lies in this block, two per thread in flight
   R vm[3] = {0, 0, 0};
-  for (int c = wave; c < nC; c += nw) {
-    const int ci = clo[0] + c % ncx, cj = clo[1] + (c / ncx) % ncy, ck = clo[2] + c / (ncx * ncy);
-    const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
-    const int cnt = rhoc[cell];
-    const int64_t base = off[cell];
-    const int ijk[3] = {ci, cj, ck};
-    for (int l = lane; l < cnt; l += 64) {
-      const int64_t ip = base + l;
+  const int total = pre[nC];
+  for (int i0 = threadIdx.x; i0 < total; i0 += 2 * 256) {
+    int ijk[2][3];
+    int64_t ip[2];
+    double xo[2][3];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + u * 256;
+      ip[u] = -1;
+      if (i < total) {
+        const int c = find_cell(pre, nC, i);
+        ijk[u][0] = clo[0] + c % ncx;
+        ijk[u][1] = clo[1] + (c / ncx) % ncy;
+        ijk[u][2] = clo[2] + c / (ncx * ncy);
+        ip[u] = gof[c] + (i - pre[c]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xo[u][d] = XP<IZ>::off(xp, ip[u], d);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (ip[u] < 0) continue;
       R w[3][3];
       int f[3];
       bool mine = true;
       for (int d = 0; d < 3; ++d) {
-        double xpos = ((double)(ijk[d] - 1) + XP<IZ>::off(xp, ip, d)) * (double)L.rcs / L.gridsize - (double)B.ushift[d];
+        double xpos = ((double)(ijk[u][d] - 1) + xo[u][d]) * (double)L.rcs / L.gridsize - (double)B.ushift[d];
         f[d] = tsc_weights<R>(xpos, w[d]) - f0[d];
         mine = mine && f[d] >= 0 && f[d] < BD;
       }
@@ -297,9 +393,9 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
               dv[d] = dv[d] + ((lds[d * FS * FS * FS + li] * w[0][i1]) * w[1][i2]) * w[2][i3];
           }
       for (int d = 0; d < 3; ++d) {
-        float v = vp[3 * ip + d];
+        float v = vp[3 * ip[u] + d];
         v = (float)kick_update((R)v, dv[d], a_mid, dt);
-        vp[3 * ip + d] = v;
+        vp[3 * ip[u] + d] = v;
         vm[d] = max(vm[d], (R)fabsf(v));
       }
     }
@@ -460,9 +556,10 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sums[nb];
 }
 
-// Block edge: 16 fine cells for the tile/sub-tile levels (FP32), 8 for the
-// coarse level (one mesh cell per coarse cell: 8^3 cells already hold ~32k
-// particles) and for the FP64 form (LDS size).
+// Deposit block edge: 16 fine cells for the tile/sub-tile levels (FP32), 8
+// for the coarse level (one mesh cell per coarse cell: 8^3 cells already hold
+// ~32k particles) and for the FP64 form (LDS size).  Interpolation blocks
+// are 8^3 force cells (LDS for phi + force of 14^3 and 3*10^3 values).
 template <typename R, int IZ, int BD>
 void launch_deposit_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
                        const int64_t* off, R* mesh, float mass, PMOut* out, int testrho, hipStream_t s) {
@@ -495,10 +592,7 @@ void launch_interp_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void
 template <typename R, int IZ>
 void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
                    const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
-  if (std::is_same<R, float>::value && L.w > 1)
-    launch_interp_bd<R, IZ, 16>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
-  else
-    launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
+  launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
 }
 
 }  // namespace

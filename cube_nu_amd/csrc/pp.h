@@ -9,7 +9,8 @@ struct PPLevel {
   int npgrid;      // ntt * rcp
   int ntt, nte, ncb, rcs;
   float pp_range, pp_range2, app;
-  float c_app, c_range;  // (1./35.)/apm**2 for app and pp_range
+  float c_app, c_range;      // (1./35.)/apm**2 for app and pp_range
+  float inv_app, inv_range;  // 1/app, 1/pp_range
   float mass_scale, v_scale;
 };
 
@@ -33,7 +34,6 @@ struct PPScratch {
   int64_t capacity = 0;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
-  int* ovf = nullptr;          // [0] count, [1..] brick ids for the global path
 };
 
 // Bin the particles of each sub-tile (plus its ghost ring) into its PP grid.
@@ -42,9 +42,6 @@ int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nb
                      const int32_t* rhoc, const int64_t* cell_off, PPScratch& S, hipStream_t s);
 void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nbox, const void* xp, const int32_t* rhoc,
                  const int64_t* cell_off, PPScratch& S, hipStream_t s);
-void pp_force(const PPLevel& L, const PPBox* boxes_dev, int nbox, float* vp, PPScratch& S, PPOut* out,
-              hipStream_t s);
-// Number of force bricks of a batch (size of the overflow list).
-int64_t pp_bricks(const PPLevel& L, int nbox);
+void pp_force(const PPLevel& L, int precision, int nbox, float* vp, PPScratch& S, PPOut* out, hipStream_t s);
 
 }  // namespace cube

@@ -795,6 +795,47 @@ int orc_fft1d(int n, double* data, int sign) {
   return 0;
 }
 
+// Decode of n*3 compressed coordinates (parameters.f90:91-92,
+// particle_mesh.f90:36) into cell fractions.
+int orc_decode(int izipx, int64_t n, const void* xp, double* out) {
+  OParams p{};
+  p.ng = 4; p.nnt = 1; p.nns = 1; p.ratio_cs = 1; p.izipx = izipx;
+  Geo g(p);
+  for (int64_t ip = 0; ip < n; ++ip)
+    for (int d = 0; d < 3; ++d) out[3 * ip + d] = decode(xp, ip, d, g);
+  return 0;
+}
+
+// TSC weights and floor index of n positions (particle_mesh.f90:56-62).
+int orc_tsc(int64_t n, const double* xpos, int64_t* idx, float* w) {
+  for (int64_t i = 0; i < n; ++i) idx[i] = tsc(xpos[i], w + 3 * i);
+  return 0;
+}
+
+// FD gradient of a periodic n^3 potential phi[k][j][i] (cells 1..n), the
+// 4-point stencil of particle_mesh.f90:237-242 over the interior 1..n:
+// force[k][j][i][d], plus f2max (particle_mesh.f90:144).
+int orc_fd_gradient(int64_t n, const float* phi, float* force, float* f2max) {
+  auto PH = [&](int64_t i, int64_t j, int64_t k) -> float {
+    i = ((i - 1) % n + n) % n; j = ((j - 1) % n + n) % n; k = ((k - 1) % n + n) % n;
+    return phi[(k * n + j) * n + i];
+  };
+  ForceField F;
+  fd_gradient(n, PH, F, f2max);
+  for (int64_t k = 1; k <= n; ++k)
+    for (int64_t j = 1; j <= n; ++j)
+      for (int64_t i = 1; i <= n; ++i)
+        for (int d = 0; d < 3; ++d) force[(((k - 1) * n + (j - 1)) * n + (i - 1)) * 3 + d] = F.at(d, i, j, k);
+  return 0;
+}
+
+// Periodic Poisson solve of one mesh as the PM levels do it (r2c, times the
+// real tf in FP32, c2r, unnormalised).
+int orc_poisson(int n, const float* rho, const float* tf, float* phi) {
+  poisson(n, rho, tf, phi);
+  return 0;
+}
+
 // Classification and napm (kick.f90:108-133).
 int orc_classify(const OParams* P, const int32_t* rhoc, int32_t* ires, int32_t napm[7]) {
   Geo g(*P);

@@ -101,6 +101,43 @@ def fft1d(x, sign=-1):
     return x
 
 
+def decode(xp, izipx):
+    """Cell fractions of compressed coordinates xp (..., 3) int8/int16."""
+    xp = np.ascontiguousarray(xp)
+    out = np.zeros(xp.shape, np.float64)
+    lib().orc_decode(C.c_int(izipx), C.c_int64(xp.size // 3), _p(xp), _p(out))
+    return out
+
+
+def tsc(xpos):
+    """(floor index int64, weights float32 (n, 3)) of FP64 positions."""
+    xpos = np.ascontiguousarray(xpos, np.float64)
+    idx = np.zeros(xpos.size, np.int64)
+    w = np.zeros((xpos.size, 3), np.float32)
+    lib().orc_tsc(C.c_int64(xpos.size), _p(xpos), _p(idx), _p(w))
+    return idx, w
+
+
+def fd_gradient(phi):
+    """Force [k][j][i][d] and f2max of a periodic potential phi [k][j][i]."""
+    n = phi.shape[0]
+    phi = np.ascontiguousarray(phi, np.float32)
+    force = np.zeros(phi.shape + (3,), np.float32)
+    f2 = C.c_float(0)
+    lib().orc_fd_gradient(C.c_int64(n), _p(phi), _p(force), C.byref(f2))
+    return force, f2.value
+
+
+def poisson(rho, tf):
+    """Unnormalised periodic solve: c2r(r2c(rho) * tf) in the PM levels' rounding order."""
+    n = rho.shape[0]
+    rho = np.ascontiguousarray(rho, np.float32)
+    tf = np.ascontiguousarray(tf, np.float32)
+    phi = np.zeros_like(rho)
+    lib().orc_poisson(C.c_int(n), _p(rho), _p(tf), _p(phi))
+    return phi
+
+
 def f_ra(r, apm):
     r = np.ascontiguousarray(r, np.float32)
     out = np.zeros_like(r)

@@ -115,6 +115,21 @@ def cpu_baseline(ng_sample, z, box):
                       f"same Zel'dovich recipe, {t:.2f} s"}
 
 
+def copy_calibration(mib):
+    """Three device-to-device copies of a known size: reading FETCH_SIZE and
+    WRITE_SIZE for them gives the counters' bytes-per-byte factor."""
+    import torch
+
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,10 +139,16 @@ def main():
     ap.add_argument("--z", type=float, default=2.0)
     ap.add_argument("--box", type=float, default=600.0)
     ap.add_argument("--fp64-steps", type=int, default=1, help="steps of the plain FP64 form (0 = skip)")
-    ap.add_argument("--cpu-sample-ng", type=int, default=128, help="0 = skip the CPU baseline")
+    ap.add_argument("--cpu-sample-ng", type=int, default=256, help="0 = skip the CPU baseline")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run device-to-device copies of this size (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it (see DESIGN.md, sharding)"}))
+        print(json.dumps({"error": f"--gpus {args.gpus}: this benchmark runs the one-image step on one GPU only; "
+                                   "a multi-GPU (nn>1) decomposition is not implemented"}))
         sys.exit(2)
     r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box)
     value = r32["n"] * args.steps / r32["seconds"]

@@ -1,0 +1,105 @@
+! cube_p3m_driver.f90 -- a one-image Fortran host that calls the step through
+! the ISO_C_BINDING interface, the way kick.f90 / drift.f90 callers would.
+!
+! Usage: cube_p3m_driver <dir> <ng> <np> <a_mid> <dt> <dt_old> <dt_drift>
+!   or   cube_p3m_driver --version | --sizes
+!
+! Reads the buffered layout from raw stream files in <dir> (the format of the
+! reference's checkpoint streams, checkpoint.f90:11-33): xp.bin (int16,
+! 3*np), vp.bin (real32, 3*np), pid.bin (int64, np), rhoc.bin (int32), then
+!   1. kick (cube_p3m_kick) and writes vp_kick.bin and kick.txt,
+!   2. drift with the kicked velocities (cube_p3m_drift) and writes
+!      xp_drift.bin, vp_drift.bin, pid_drift.bin, rhoc_drift.bin,
+!      spine.bin (idx_b_l, idx_b_r, ppl0, pplr, pprl, ppr0, ppl, ppr) and
+!      drift.txt.
+! Exit status 0 on success; 3 when no GPU is visible.
+program cube_p3m_driver
+  use iso_c_binding
+  use cube_p3m_iface
+  implicit none
+  character(512) :: dir, arg
+  integer :: ng, nc, nt, nte, nnt, ncb, u
+  integer(c_int64_t) :: np, nrc, nsp, nsb
+  real(c_float) :: a_mid, dt, dt_old, dt_drift
+  integer(c_int16_t), allocatable :: xp(:)
+  real(c_float), allocatable :: vp(:)
+  integer(c_int64_t), allocatable :: pid(:), idx_b_l(:), idx_b_r(:), sp(:, :)
+  integer(c_int32_t), allocatable :: rhoc(:)
+  type(cube_p3m_config) :: cfg
+  type(c_ptr) :: ctx
+  type(cube_kick_in) :: kin
+  type(cube_kick_out) :: kout
+  type(cube_drift_in) :: din
+  type(cube_drift_out) :: dout
+  integer(c_int) :: st
+
+  call get_command_argument(1, arg)
+  if (trim(arg) == '--version') then
+    print '(a)', c_string(cube_p3m_version())
+    stop
+  endif
+  if (trim(arg) == '--sizes') then  ! bind(C) type sizes, to check against the C structs
+    print '(6i6)', c_sizeof(cfg), c_sizeof(kin), c_sizeof(kout), c_sizeof(din), c_sizeof(dout), &
+        c_sizeof(cube_ic_params(0, 0d0, 0d0, 0d0, 0d0, 0d0, 0d0, 0, 0.0))
+    stop
+  endif
+  dir = arg
+  call get_command_argument(2, arg); read(arg, *) ng
+  call get_command_argument(3, arg); read(arg, *) np
+  call get_command_argument(4, arg); read(arg, *) a_mid
+  call get_command_argument(5, arg); read(arg, *) dt
+  call get_command_argument(6, arg); read(arg, *) dt_old
+  call get_command_argument(7, arg); read(arg, *) dt_drift
+
+  cfg = cube_default_config(ng, 600.0 * ng / 512.0)
+  nnt = cfg%nnt; ncb = cfg%ncb
+  nc = ng / cfg%ratio_cs; nt = nc / nnt; nte = nt + 2 * ncb
+  nrc = int(nte, c_int64_t)**3 * nnt**3
+  nsb = int(nte, c_int64_t)**2 * nnt**3
+  nsp = int(nt, c_int64_t)**2 * nnt**3
+
+  st = cube_p3m_create(cfg, ctx)
+  if (st == CUBE_ERR_NO_DEVICE) then
+    print '(a)', 'no GPU visible'
+    call exit(3)
+  endif
+  call cube_check(st, ctx, 'cube_p3m_create')
+
+  allocate(xp(3 * np), vp(3 * np), pid(np), rhoc(nrc), idx_b_l(nsb), idx_b_r(nsb), sp(nsp, 6))
+  open(newunit=u, file=trim(dir)//'/xp.bin', access='stream', form='unformatted', status='old'); read(u) xp; close(u)
+  open(newunit=u, file=trim(dir)//'/vp.bin', access='stream', form='unformatted', status='old'); read(u) vp; close(u)
+  open(newunit=u, file=trim(dir)//'/pid.bin', access='stream', form='unformatted', status='old'); read(u) pid; close(u)
+  open(newunit=u, file=trim(dir)//'/rhoc.bin', access='stream', form='unformatted', status='old'); read(u) rhoc
+  close(u)
+  open(newunit=u, file=trim(dir)//'/idx_b_r.bin', access='stream', form='unformatted', status='old'); read(u) idx_b_r
+  close(u)
+
+  kin%a_mid = a_mid; kin%dt = dt; kin%mass_p_cdm = 1.0; kin%phase_mask = 0
+  call cube_check(cube_p3m_kick(ctx, xp, vp, rhoc, idx_b_r, np, kin, kout), ctx, 'kick')
+  open(newunit=u, file=trim(dir)//'/vp_kick.bin', access='stream', form='unformatted', status='replace')
+  write(u) vp; close(u)
+  open(newunit=u, file=trim(dir)//'/kick.txt', status='replace')
+  write(u, '(6es24.16)') kout%dt_pm1, kout%dt_pm2, kout%dt_pm3, kout%dt_pp, kout%dt_vmax, kout%vz_max
+  write(u, '(7i12)') kout%napm
+  write(u, '(es24.16)') kout%testrho
+  close(u)
+
+  din%dt_old = dt_old; din%dt = dt_drift; din%vz_max = kout%vz_max
+  call cube_check(cube_p3m_drift(ctx, xp, vp, pid, rhoc, np, idx_b_l, idx_b_r, sp(:, 1), sp(:, 2), sp(:, 3), &
+                                 sp(:, 4), sp(:, 5), sp(:, 6), din, dout), ctx, 'drift')
+  open(newunit=u, file=trim(dir)//'/xp_drift.bin', access='stream', form='unformatted', status='replace')
+  write(u) xp(1:3 * dout%nplocal); close(u)
+  open(newunit=u, file=trim(dir)//'/vp_drift.bin', access='stream', form='unformatted', status='replace')
+  write(u) vp(1:3 * dout%nplocal); close(u)
+  open(newunit=u, file=trim(dir)//'/pid_drift.bin', access='stream', form='unformatted', status='replace')
+  write(u) pid(1:dout%nplocal); close(u)
+  open(newunit=u, file=trim(dir)//'/rhoc_drift.bin', access='stream', form='unformatted', status='replace')
+  write(u) rhoc; close(u)
+  open(newunit=u, file=trim(dir)//'/spine.bin', access='stream', form='unformatted', status='replace')
+  write(u) idx_b_l, idx_b_r, sp; close(u)
+  open(newunit=u, file=trim(dir)//'/drift.txt', status='replace')
+  write(u, '(i16, es24.16, i8)') dout%nplocal, dout%overhead_tile, dout%nlayer
+  close(u)
+
+  call cube_check(cube_p3m_destroy(ctx), c_null_ptr, 'cube_p3m_destroy')
+end program

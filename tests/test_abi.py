@@ -42,3 +42,24 @@ def test_create_without_device_or_bad_config():
         lib.cube_p3m_destroy(ctx)
         pytest.skip("a HIP device is present")
     assert rc == CUBE_ERR_NO_DEVICE
+
+
+DRIVER = os.path.join(ROOT, "cube_nu_amd", "lib", "cube_p3m_driver")
+
+
+def test_fortran_bindc_types_match_c_structs():
+    """The ISO_C_BINDING derived types have the C structs' sizes."""
+    import subprocess
+
+    from cube_nu_amd.p3m import DriftIn, DriftOut, ICParams, KickIn, KickOut
+
+    out = subprocess.run([DRIVER, "--sizes"], capture_output=True, text=True, check=True).stdout.split()
+    sizes = [int(v) for v in out[:6]]
+    assert sizes == [ctypes.sizeof(c) for c in (Config, KickIn, KickOut, DriftIn, DriftOut, ICParams)]
+
+
+def test_fortran_driver_links_the_library():
+    import subprocess
+
+    r = subprocess.run([DRIVER, "--version"], capture_output=True, text=True, check=True)
+    assert "gfx950" in r.stdout

@@ -79,6 +79,7 @@ struct XP<1> {
   __device__ static double off(const int8_t* xp, int64_t ip, int d) {
     return ((double)(uint32_t)(uint8_t)xp[3 * ip + d] + 0.5) * (1.0 / 256.0);
   }
+  __device__ static uint32_t raw(const int8_t* xp, int64_t ip, int d) { return (uint8_t)xp[3 * ip + d]; }
   static constexpr int B = 8;
 };
 template <>
@@ -87,6 +88,7 @@ struct XP<2> {
   __device__ static double off(const int16_t* xp, int64_t ip, int d) {
     return ((double)(uint32_t)(uint16_t)xp[3 * ip + d] + 0.5) * (1.0 / 65536.0);
   }
+  __device__ static uint32_t raw(const int16_t* xp, int64_t ip, int d) { return (uint16_t)xp[3 * ip + d]; }
   static constexpr int B = 16;
 };
 

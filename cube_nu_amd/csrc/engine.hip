@@ -91,6 +91,7 @@ struct cube_p3m_ctx {
   // scratch
   DevBuf<char> scan_tmp;
   DevBuf<char> work;
+  DevBuf<char> stencil;  // coarse-level deposit: 27 tap sums per coarse cell
   DevBuf<PMBox> boxes;
   DevBuf<PPBox> ppboxes;
   DevBuf<PMOut> pmout;
@@ -237,11 +238,16 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
     CUBE_HIP(hipMemcpyAsync(boxes.p, bx.data(), sizeof(PMBox) * nb, hipMemcpyHostToDevice, s));
     PMLevel L = L0;
     L.bstride = nreal;
+    pm_level_scale(L);
     const int nmesh = (level == 1) ? 1 : nb;
     const double cells = (double)nreal * nmesh;
     {
       KScope k(&tm, (t + "deposit").c_str(), cells * sizeof(R));
-      pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, o, s);
+      if (level == 1)
+        pm1_deposit<R>((int)g.izipx, L, xpc(), rhocc(), off.p, stencil.ensure(pm1_stencil_bytes(L)), mesh,
+                       in.mass_p_cdm, o, s);
+      else
+        pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, s);
     }
     if (density_host) {
       std::vector<R> tmp(nreal);
@@ -430,7 +436,7 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
       tfx += po[0].testrho_fx[q];
       td += po[0].testrho_d[q];
     }
-    out.testrho = std::is_same<R, float>::value ? (double)tfx / 1099511627776.0 : td;
+    out.testrho = std::is_same<R, float>::value ? (double)tfx / 4294967296.0 : td;
     for (int d = 0; d < 3; ++d) vmax[d] = vm(po[0], d);
   }
   if (mask & 2) {

@@ -17,6 +17,8 @@ struct PMLevel {
   int w;             // fine mesh cells per coarse cell (integer, for bounds)
   int rcs;           // ratio_cs
   double gridsize;   // pm%gridsize as stored (real(4) promoted)
+  double xmul;       // ratio_cs / gridsize (exact when gridsize is a power of two)
+  int xdiv;          // 1: divide by gridsize (not a power of two), 0: multiply by xmul
   int nforce;        // pm%nforce: physical force cells 1..nforce
   int nte, ncb, nt;  // tile geometry for rhoc indexing
 };
@@ -35,13 +37,24 @@ struct PMOut {  // device-side reductions of one level
   float vmax[3];
   double f2max_d;
   double vmax_d[3];
-  unsigned long long testrho_fx[kSumSlots];  // PM1 density sum in 2^-40 fixed point
+  unsigned long long testrho_fx[kSumSlots];  // PM1 density sum in 2^-32 fixed point
   double testrho_d[kSumSlots];
 };
 
+// Tile/sub-tile (fine) level deposit into each box's mesh.
 template <typename R>
 void pm_deposit(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp, const int32_t* rhoc,
-                const int64_t* cell_off, R* mesh, float mass, PMOut* out, hipStream_t s);
+                const int64_t* cell_off, R* mesh, float mass, hipStream_t s);
+
+// Coarse (PM1) deposit over all tiles into the periodic nc^3 mesh, with the
+// exact testrho sum; `stencil` holds pm1_stencil_bytes(L) of scratch.
+template <typename R>
+void pm1_deposit(int izipx, const PMLevel& L, const void* xp, const int32_t* rhoc, const int64_t* cell_off,
+                 void* stencil, R* mesh, float mass, PMOut* out, hipStream_t s);
+size_t pm1_stencil_bytes(const PMLevel& L);
+
+// Fills xmul/xdiv from rcs and gridsize.
+void pm_level_scale(PMLevel& L);
 
 template <typename R>
 void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp, float* vp,

@@ -4,9 +4,11 @@
 // particles of every coarse cell whose TSC footprint reaches the block
 // (particles are cell-sorted, so these are contiguous runs), one wavefront
 // per coarse cell, and accumulates (wx*wy)*wz into LDS.  The FP32 form sums
-// in 2^-40 fixed point with 64-bit integer LDS atomics: integer addition is
-// associative, so the result does not depend on thread order and is more
-// accurate than an FP32 running sum.  The FP64 form uses FP64 LDS atomics.
+// in 2^-32 fixed point with 64-bit integer LDS atomics: each FP32 weight
+// (< 0.43) becomes one 32-bit integer (v_cvt_u32_f32 after rounding), integer
+// addition is associative, so the result does not depend on thread order and
+// is more accurate than an FP32 running sum.  The FP64 form uses FP64 LDS
+// atomics.
 //
 // Interpolation: one workgroup per BD^3 block of force cells.  The block
 // stages phi (block + 3-cell halo) in LDS, evaluates the 4-point FD force
@@ -14,6 +16,7 @@
 // over its physical cells, and applies the 27-point TSC gather and kick to
 // the particles whose floor(xpos) lies in the block.  The force field is
 // never written to HBM.
+#include <cmath>
 #include <type_traits>
 
 #include "pm.h"
@@ -22,7 +25,10 @@
 namespace cube {
 namespace {
 
-constexpr double kFix = 1099511627776.0;  // 2^40
+constexpr float kFixF = 4294967296.0f;  // 2^32: weights are < 0.43, so w * 2^32 fits a u32
+
+// FP32 TSC weight product as a 2^-32 fixed-point integer (round to nearest).
+__device__ inline unsigned long long fix32(float cw) { return (unsigned long long)__float2uint_rn(cw * kFixF); }
 
 __device__ inline int floor_div(int a, int b) {
   int q = a / b;
@@ -97,16 +103,68 @@ __device__ inline int find_cell(const int* pre, int nC, int i) {
   return lo;
 }
 
+// Position of a particle in the box's mesh units, particle_mesh.f90:36-37 /
+// kick.f90:628-629: ((c-1) + frac) * ratio_cs / gridsize - shift in FP64.
+// When gridsize is a power of two the product with ratio_cs/gridsize is the
+// same double as the division.
+__device__ inline double pm_xpos(const PMLevel& L, int c, double frac, int shift) {
+  const double a = (double)(c - 1) + frac;
+  return (L.xdiv ? (a * (double)L.rcs) / L.gridsize : a * L.xmul) - (double)shift;
+}
+
+// Deposits one particle's 27 TSC weights into the block's LDS accumulator.
+template <typename R, typename Acc, int BD>
+__device__ inline void deposit_particle(Acc* acc, const double xpos[3], const int mlo[3], const int mhi[3]) {
+  R w[3][3];
+  int f[3];
+  for (int d = 0; d < 3; ++d) f[d] = tsc_weights<R>(xpos[d], w[d]);
+  const bool inner = f[0] >= mlo[0] && f[0] + 2 <= mhi[0] && f[1] >= mlo[1] && f[1] + 2 <= mhi[1] &&
+                     f[2] >= mlo[2] && f[2] + 2 <= mhi[2];
+  for (int i3 = 0; i3 < 3; ++i3) {
+    const int mz = f[2] + i3;
+    if (!inner && (mz < mlo[2] || mz > mhi[2])) continue;
+    for (int i2 = 0; i2 < 3; ++i2) {
+      const int my = f[1] + i2;
+      if (!inner && (my < mlo[1] || my > mhi[1])) continue;
+      const R wyz = w[1][i2];
+      const int row = ((mz - mlo[2]) * BD + (my - mlo[1])) * BD - mlo[0];
+      for (int i1 = 0; i1 < 3; ++i1) {
+        const int mx = f[0] + i1;
+        if (!inner && (mx < mlo[0] || mx > mhi[0])) continue;
+        const R cw = (w[0][i1] * wyz) * w[2][i3];  // dx(1,i1)*dx(2,i2)*dx(3,i3)
+        if constexpr (std::is_same<R, float>::value) atomicAdd(&acc[row + mx], fix32(cw));
+        else atomicAdd(&acc[row + mx], cw);
+      }
+    }
+  }
+}
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Fine-level deposit (tile and sub-tile meshes), gather form: the block owns
+// BD^3 mesh cells and reads every coarse cell whose particles can reach them.
+// Only a thin layer of a ring cell reaches the block (1/w of its particles per
+// axis), so each wavefront loads the compressed coordinates of four cells'
+// particles at a time (twelve loads in flight per lane), tests the floor
+// index (integer arithmetic on the coordinate when ratio_cs/gridsize is a
+// power of two), appends hits with their coordinates to a per-wavefront LDS
+// queue, and deposits 64 queued particles at a time with every lane busy and
+// no memory access but LDS atomics.
 template <typename R, int IZ, int BD>
 __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restrict__ boxes,
                                                  const XT<IZ>* __restrict__ xp, const int32_t* __restrict__ rhoc,
-                                                 const int64_t* __restrict__ off, R* __restrict__ mesh, float mass,
-                                                 PMOut* __restrict__ out, int testrho) {
+                                                 const int64_t* __restrict__ off, R* __restrict__ mesh, float mass) {
   using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
+  constexpr int kMaxFine = 216;  // (16/4 + 2)^3: fine levels have w >= 4 (checked on the host)
+  constexpr int G = 4;           // cells loaded together per wavefront
   __shared__ Acc acc[BD * BD * BD];
-  __shared__ Acc red[256 / 64];
-  __shared__ int cnt[kMaxCells], pre[kMaxCells + 1], wsum[4];
-  __shared__ int64_t gof[kMaxCells];
+  __shared__ int cnt[kMaxFine], pre[kMaxFine + 1], wsum[4];
+  __shared__ int64_t gof[kMaxFine];
+  __shared__ unsigned long long queue[4][128];
   const int nblk = (L.win_hi - L.win_lo + 1 + BD - 1) / BD;
   int64_t b = blockIdx.x;
   const int bx = (int)(b % nblk);
@@ -129,123 +187,175 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   const int nC = stage_cells(L, B, clo, chi, rhoc, off, cnt, pre, gof, wsum);
   const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  if (L.w == 1) {
-    // Coarse mesh: every particle of cell (ci,cj,ck) has floor(xpos) = ci-1
-    // etc., so all of them hit the same 27 mesh cells.  Lanes keep 27 partial
-    // sums, the wavefront reduces them in a fixed tree, and one lane adds
-    // each to LDS.
-    for (int c = wave; c < nC; c += nw) {
-      const int n = cnt[c];
-      if (n == 0) continue;
-      const int ci = clo[0] + c % ncx, cj = clo[1] + (c / ncx) % ncy, ck = clo[2] + c / (ncx * ncy);
-      const int64_t base = gof[c];
-      const int ijk[3] = {ci, cj, ck};
-      Acc a27[27];
-#pragma unroll
-      for (int q = 0; q < 27; ++q) a27[q] = 0;
-      for (int l = lane; l < n; l += 64) {
-        R w[3][3];
-        for (int d = 0; d < 3; ++d) {
-          double xpos = ((double)(ijk[d] - 1) + XP<IZ>::off(xp, base + l, d)) * (double)L.rcs / L.gridsize -
-                        (double)B.ushift[d];
-          tsc_weights<R>(xpos, w[d]);
-        }
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-          const R cw = (w[0][q % 3] * w[1][(q / 3) % 3]) * w[2][q / 9];
-          if constexpr (std::is_same<R, float>::value) a27[q] += (Acc)__double2ull_rn((double)cw * kFix);
-          else a27[q] += cw;
-        }
+  unsigned long long* q = queue[wave];
+  // the floor index as integer arithmetic: shift = B - log2(w)
+  const int sh = L.xdiv ? -1 : XP<IZ>::B - (31 - __clz(L.w));
+  constexpr double kRes = 1.0 / (double)(1u << XP<IZ>::B);
+  auto process = [&](int nq) {
+    wave_sync_lds();
+    if (lane < nq) {
+      const unsigned long long e = q[lane];
+      const int c = (int)(e >> 48);
+      const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+      double xpos[3];
+      for (int d = 0; d < 3; ++d) {
+        const uint32_t u = (uint32_t)(e >> (16 * d)) & 0xFFFFu;
+        xpos[d] = pm_xpos(L, ijk[d], ((double)u + 0.5) * kRes, B.ushift[d]);
       }
-#pragma unroll
-      for (int q = 0; q < 27; ++q) {
-        Acc v = a27[q];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        // taps floor(xpos) + {0,1,2} = ci-1, ci, ci+1 (particle_mesh.f90:56-58)
-        const int mx = ci - 1 + q % 3, my = cj - 1 + (q / 3) % 3, mz = ck - 1 + q / 9;
-        if (lane == 0 && mx >= mlo[0] && mx <= mhi[0] && my >= mlo[1] && my <= mhi[1] && mz >= mlo[2] &&
-            mz <= mhi[2])
-          atomicAdd(&acc[((mz - mlo[2]) * BD + (my - mlo[1])) * BD + (mx - mlo[0])], v);
-      }
+      deposit_particle<R, Acc, BD>(acc, xpos, mlo, mhi);
     }
-  } else {
-    // Fine meshes: particles of all the block's cells in one flat loop,
-    // four per thread in flight.
-    const int total = pre[nC];
-    for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
-      int ijk[4][3];
-      int64_t ip[4];
-      double xo[4][3];
+  };
+  int qn = 0;  // wave-uniform queue length
+  for (int c0 = wave; c0 < nC; c0 += G * nw) {
+    int nmax = 0;
+    for (int k = 0; k < G; ++k) {
+      const int c = c0 + k * nw;
+      if (c < nC) nmax = max(nmax, cnt[c]);
+    }
+    for (int l0 = 0; l0 < nmax; l0 += 64) {
+      const int l = l0 + lane;
+      uint32_t u[G][3];
+      bool valid[G];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * 256;
-        ip[u] = -1;
-        if (i < total) {
-          const int c = find_cell(pre, nC, i);
-          ijk[u][0] = clo[0] + c % ncx;
-          ijk[u][1] = clo[1] + (c / ncx) % ncy;
-          ijk[u][2] = clo[2] + c / (ncx * ncy);
-          ip[u] = gof[c] + (i - pre[c]);
-#pragma unroll
-          for (int d = 0; d < 3; ++d) xo[u][d] = XP<IZ>::off(xp, ip[u], d);
-        }
+      for (int k = 0; k < G; ++k) {
+        const int c = c0 + k * nw;
+        valid[k] = c < nC && l < cnt[c];
+        const int64_t ip = valid[k] ? gof[c] + l : 0;
+        for (int d = 0; d < 3; ++d) u[k][d] = valid[k] ? XP<IZ>::raw(xp, ip, d) : 0u;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (ip[u] < 0) continue;
-        R w[3][3];
-        int f[3];
-        for (int d = 0; d < 3; ++d) {
-          double xpos = ((double)(ijk[u][d] - 1) + xo[u][d]) * (double)L.rcs / L.gridsize - (double)B.ushift[d];
-          f[d] = tsc_weights<R>(xpos, w[d]);
-        }
-        for (int i3 = 0; i3 < 3; ++i3) {
-          const int mz = f[2] + i3;
-          if (mz < mlo[2] || mz > mhi[2]) continue;
-          for (int i2 = 0; i2 < 3; ++i2) {
-            const int my = f[1] + i2;
-            if (my < mlo[1] || my > mhi[1]) continue;
-            const R wyz = w[1][i2];
-            for (int i1 = 0; i1 < 3; ++i1) {
-              const int mx = f[0] + i1;
-              if (mx < mlo[0] || mx > mhi[0]) continue;
-              const R cw = (w[0][i1] * wyz) * w[2][i3];  // dx(1,i1)*dx(2,i2)*dx(3,i3)
-              const int li = ((mz - mlo[2]) * BD + (my - mlo[1])) * BD + (mx - mlo[0]);
-              if constexpr (std::is_same<R, float>::value)
-                atomicAdd(&acc[li], (unsigned long long)__double2ull_rn((double)cw * kFix));
-              else
-                atomicAdd(&acc[li], cw);
-            }
+      for (int k = 0; k < G; ++k) {
+        const int c = c0 + k * nw;
+        bool hit = valid[k];
+        if (hit) {
+          const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+          for (int d = 0; d < 3; ++d) {
+            int f;
+            if (sh >= 0) f = L.w * (ijk[d] - 1) - B.ushift[d] + (int)(u[k][d] >> sh);
+            else f = (int)floor(pm_xpos(L, ijk[d], ((double)u[k][d] + 0.5) * kRes, B.ushift[d]));
+            hit = hit && f + 2 >= mlo[d] && f <= mhi[d];
           }
+        }
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+          const int pos = qn + (int)__popcll(m & ((1ull << lane) - 1ull));
+          q[pos] = ((unsigned long long)c << 48) | ((unsigned long long)u[k][2] << 32) |
+                   ((unsigned long long)u[k][1] << 16) | (unsigned long long)u[k][0];
+        }
+        qn += (int)__popcll(m);
+        if (qn >= 64) {
+          process(64);
+          // move the overflow (< 64 entries) to the front
+          wave_sync_lds();
+          const unsigned long long mv = (lane < qn - 64) ? q[64 + lane] : 0ull;
+          wave_sync_lds();
+          if (lane < qn - 64) q[lane] = mv;
+          qn -= 64;
         }
       }
     }
   }
+  if (qn > 0) process(qn);
   __syncthreads();
-  Acc tsum = 0;
   for (int t = threadIdx.x; t < BD * BD * BD; t += blockDim.x) {
     const int lx = t % BD, ly = (t / BD) % BD, lz = t / (BD * BD);
     const int m[3] = {mlo[0] + lx, mlo[1] + ly, mlo[2] + lz};
     if (m[0] > mhi[0] || m[1] > mhi[1] || m[2] > mhi[2]) continue;
     R v;
-    if constexpr (std::is_same<R, float>::value) v = (float)((double)acc[t] * (1.0 / kFix));
+    if constexpr (std::is_same<R, float>::value) v = (float)((double)acc[t] * (1.0 / (double)kFixF));
     else v = acc[t];
-    tsum += acc[t];
     int a[3];
     for (int d = 0; d < 3; ++d) a[d] = m[d] + B.org[d] - L.m0;
     mesh[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]] = v * (R)mass;
   }
-  if (testrho) {
-    // wave then block reduction of the (exact, for FP32) integer sums
-    for (int o = 32; o > 0; o >>= 1) tsum += __shfl_down(tsum, o, 64);
-    if (lane == 0) red[wave] = tsum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      Acc s = 0;
-      for (int q = 0; q < nw; ++q) s += red[q];
-      if constexpr (std::is_same<R, float>::value) atomicAdd(&out->testrho_fx[blockIdx.x % kSumSlots], s);
-      else atomicAdd(&out->testrho_d[blockIdx.x % kSumSlots], s);
+}
+
+// Coarse-level deposit (PM1), step 1.  Every particle of coarse cell c has
+// floor(xpos) = c-1 and deposits into the same 27 mesh cells c-1..c+1
+// (particle_mesh.f90:56-58), so one wavefront sums the 27 tap weights of a
+// cell over its particles (exact fixed point, or FP64), transposes the lane
+// partials through LDS and writes the 27 sums: stencil[tap][cell].  Each
+// particle is read once.
+template <typename R, int IZ>
+__global__ void __launch_bounds__(128) k_pm1_stencil(PMLevel L, const XT<IZ>* __restrict__ xp,
+                                                     const int32_t* __restrict__ rhoc,
+                                                     const int64_t* __restrict__ off, void* __restrict__ stencil) {
+  using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
+  __shared__ Acc red[2][27][65];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ncell = (int64_t)L.n * L.n * L.n;
+  const int64_t g = (int64_t)blockIdx.x * 2 + wave;
+  if (g >= ncell) return;
+  const int nt = L.nt, nnt = L.n / L.nt;
+  const int gx = (int)(g % L.n), gy = (int)((g / L.n) % L.n), gz = (int)(g / ((int64_t)L.n * L.n));
+  const int T = ((gz / nt) * nnt + gy / nt) * nnt + gx / nt;
+  const int ijk[3] = {gx % nt + 1, gy % nt + 1, gz % nt + 1};
+  const int64_t cell = rc_index(ijk[0], ijk[1], ijk[2], T, L.nte, L.ncb);
+  const int n = rhoc[cell];
+  const int64_t base = off[cell];
+  Acc a27[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) a27[q] = 0;
+  for (int l = lane; l < n; l += 64) {
+    R w[3][3];
+    for (int d = 0; d < 3; ++d) tsc_weights<R>(pm_xpos(L, ijk[d], XP<IZ>::off(xp, base + l, d), 0), w[d]);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const R cw = (w[0][q % 3] * w[1][(q / 3) % 3]) * w[2][q / 9];
+      if constexpr (std::is_same<R, float>::value) a27[q] += fix32(cw);
+      else a27[q] += cw;
     }
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) red[wave][q][lane] = a27[q];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < 27) {
+    Acc s = 0;
+    for (int l = 0; l < 64; ++l) s += red[wave][lane][l];  // fixed order
+    ((Acc*)stencil)[(int64_t)lane * ncell + g] = s;
+  }
+}
+
+// Coarse-level deposit, step 2: mesh cell M (global, periodic) is the sum of
+// tap t of the cells M+1-t, t in {0,1,2}^3, in a fixed order; then the
+// reference's rho1 = rho * mass_p_cdm (particle_mesh.f90:102-105) and the
+// exact testrho sum (particle_mesh.f90:106).
+template <typename R>
+__global__ void __launch_bounds__(256) k_pm1_gather(PMLevel L, const void* __restrict__ stencil, R* __restrict__ mesh,
+                                                    float mass, PMOut* __restrict__ out) {
+  using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
+  __shared__ Acc red[4];
+  const int n = L.n;
+  const int64_t ncell = (int64_t)n * n * n;
+  const int64_t M = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Acc v = 0;
+  if (M < ncell) {
+    const int mx = (int)(M % n), my = (int)((M / n) % n), mz = (int)(M / ((int64_t)n * n));
+    const Acc* st = (const Acc*)stencil;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const int tx = q % 3, ty = (q / 3) % 3, tz = q / 9;
+      int cx = mx + 1 - tx, cy = my + 1 - ty, cz = mz + 1 - tz;
+      cx = cx < 0 ? cx + n : (cx >= n ? cx - n : cx);
+      cy = cy < 0 ? cy + n : (cy >= n ? cy - n : cy);
+      cz = cz < 0 ? cz + n : (cz >= n ? cz - n : cz);
+      v += st[(int64_t)q * ncell + ((int64_t)cz * n + cy) * n + cx];
+    }
+    R r;
+    if constexpr (std::is_same<R, float>::value) r = (float)((double)v * (1.0 / (double)kFixF));
+    else r = v;
+    mesh[M] = r * (R)mass;
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Acc s = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+    if constexpr (std::is_same<R, float>::value) atomicAdd(&out->testrho_fx[blockIdx.x % kSumSlots], s);
+    else atomicAdd(&out->testrho_d[blockIdx.x % kSumSlots], s);
   }
 }
 
@@ -260,7 +370,7 @@ __device__ inline R block_max(R v, R* red) {
   return m;
 }
 
-template <typename R, int IZ, int BD>
+template <typename R, int IZ, int BD, int MAXC>
 __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restrict__ boxes,
                                                 const XT<IZ>* __restrict__ xp, float* __restrict__ vp,
                                                 const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off,
@@ -269,7 +379,6 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
   constexpr int FS = BD + 2, PS = BD + 6;
   constexpr int NF = (FS * FS * FS + 255) / 256;
   constexpr int LDSN = (PS * PS * PS > 3 * FS * FS * FS) ? PS * PS * PS : 3 * FS * FS * FS;
-  constexpr int MAXC = kMaxCells;
   __shared__ R lds[LDSN];
   __shared__ R red[4];
   __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4];
@@ -291,20 +400,61 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
     clo[d] = max(B.nc1[d], floor_div(f0[d] + B.ushift[d], L.w) + 1);
     chi[d] = min(B.nc2[d], floor_div(f0[d] + BD - 1 + B.ushift[d], L.w) + 1);
   }
-  // 1. phi(f0-2 .. f0+BD+3) into LDS, cell metadata alongside
-  for (int t = threadIdx.x; t < PS * PS * PS; t += blockDim.x) {
-    const int l[3] = {t % PS, (t / PS) % PS, t / (PS * PS)};
-    int a[3];
-    bool ok = true;
-    for (int d = 0; d < 3; ++d) {
-      a[d] = f0[d] - 2 + l[d] + B.org[d] - L.m0;
-      if (L.periodic) a[d] = ((a[d] % L.n) + L.n) % L.n;
-      else if (a[d] < 0 || a[d] >= L.n) ok = false;
+  // 1. phi(f0-2 .. f0+BD+3) and the cell metadata: every load is issued
+  //    before the first LDS store, so the block waits for one round trip.
+  constexpr int NPH = (PS * PS * PS + 255) / 256;
+  R ph[NPH];
+#pragma unroll
+  for (int q = 0; q < NPH; ++q) {
+    const int t = threadIdx.x + q * 256;
+    ph[q] = (R)0;
+    if (t < PS * PS * PS) {
+      const int l[3] = {t % PS, (t / PS) % PS, t / (PS * PS)};
+      int a[3];
+      bool ok = true;
+      for (int d = 0; d < 3; ++d) {
+        a[d] = f0[d] - 2 + l[d] + B.org[d] - L.m0;
+        if (L.periodic) a[d] = a[d] < 0 ? a[d] + L.n : (a[d] >= L.n ? a[d] - L.n : a[d]);
+        else ok = ok && a[d] >= 0 && a[d] < L.n;
+      }
+      if (ok) ph[q] = phi[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]];
     }
-    lds[t] = ok ? phi[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]] : (R)0;
   }
-  const int nC = stage_cells(L, B, clo, chi, rhoc, off, cnt, pre, gof, wsum);  // contains barriers
-  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1);
+  const int ncx = max(0, chi[0] - clo[0] + 1), ncy = max(0, chi[1] - clo[1] + 1),
+            ncz = max(0, chi[2] - clo[2] + 1);
+  const int nC = ncx * ncy * ncz;
+  for (int t = threadIdx.x; t < nC; t += blockDim.x) {
+    const int64_t cell = rc_index(clo[0] + t % ncx, clo[1] + (t / ncx) % ncy, clo[2] + t / (ncx * ncy), B.T, L.nte,
+                                  L.ncb);
+    cnt[t] = rhoc[cell];
+    gof[t] = off[cell];
+  }
+#pragma unroll
+  for (int q = 0; q < NPH; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < PS * PS * PS) lds[t] = ph[q];
+  }
+  __syncthreads();
+  block_scan(cnt, pre, nC, wsum);
+  // first two particles of this thread: coordinates and velocities loaded
+  // now, consumed after the force computation
+  const int total = pre[nC];
+  int pc[2];
+  uint32_t pu[2][3];
+  float pv[2][3];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int i = threadIdx.x + u * 256;
+    pc[u] = -1;
+    if (i < total) {
+      pc[u] = find_cell(pre, nC, i);
+      const int64_t ip = gof[pc[u]] + (i - pre[pc[u]]);
+      for (int d = 0; d < 3; ++d) {
+        pu[u][d] = XP<IZ>::raw(xp, ip, d);
+        pv[u][d] = vp[3 * ip + d];
+      }
+    }
+  }
   // 2. force(d, f0 .. f0+BD+1) = sum(phi(m + [-2,-1,1,2]) * weight)
   const FDWeights<R> W;
   R fr[NF][3];
@@ -351,35 +501,38 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
     }
   }
   __syncthreads();
-  // 3. particles whose floor(xpos) lies in this block, two per thread in flight
+  // 3. particles whose floor(xpos) lies in this block
+  constexpr double kRes = 1.0 / (double)(1u << XP<IZ>::B);
   R vm[3] = {0, 0, 0};
-  const int total = pre[nC];
   for (int i0 = threadIdx.x; i0 < total; i0 += 2 * 256) {
-    int ijk[2][3];
-    int64_t ip[2];
-    double xo[2][3];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int i = i0 + u * 256;
-      ip[u] = -1;
-      if (i < total) {
-        const int c = find_cell(pre, nC, i);
-        ijk[u][0] = clo[0] + c % ncx;
-        ijk[u][1] = clo[1] + (c / ncx) % ncy;
-        ijk[u][2] = clo[2] + c / (ncx * ncy);
-        ip[u] = gof[c] + (i - pre[c]);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) xo[u][d] = XP<IZ>::off(xp, ip[u], d);
+      if (i >= total) continue;
+      int c;
+      uint32_t uu[3];
+      float v0[3];
+      if (i0 == (int)threadIdx.x) {
+        c = pc[u];
+        for (int d = 0; d < 3; ++d) {
+          uu[d] = pu[u][d];
+          v0[d] = pv[u][d];
+        }
+      } else {
+        c = find_cell(pre, nC, i);
+        const int64_t ipl = gof[c] + (i - pre[c]);
+        for (int d = 0; d < 3; ++d) {
+          uu[d] = XP<IZ>::raw(xp, ipl, d);
+          v0[d] = vp[3 * ipl + d];
+        }
       }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (ip[u] < 0) continue;
+      const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+      const int64_t ip = gof[c] + (i - pre[c]);
       R w[3][3];
       int f[3];
       bool mine = true;
       for (int d = 0; d < 3; ++d) {
-        double xpos = ((double)(ijk[u][d] - 1) + xo[u][d]) * (double)L.rcs / L.gridsize - (double)B.ushift[d];
+        const double xpos = pm_xpos(L, ijk[d], ((double)uu[d] + 0.5) * kRes, B.ushift[d]);
         f[d] = tsc_weights<R>(xpos, w[d]) - f0[d];
         mine = mine && f[d] >= 0 && f[d] < BD;
       }
@@ -393,9 +546,8 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
               dv[d] = dv[d] + ((lds[d * FS * FS * FS + li] * w[0][i1]) * w[1][i2]) * w[2][i3];
           }
       for (int d = 0; d < 3; ++d) {
-        float v = vp[3 * ip[u] + d];
-        v = (float)kick_update((R)v, dv[d], a_mid, dt);
-        vp[3 * ip[u] + d] = v;
+        const float v = (float)kick_update((R)v0[d], dv[d], a_mid, dt);
+        vp[3 * ip + d] = v;
         vm[d] = max(vm[d], (R)fabsf(v));
       }
     }
@@ -562,21 +714,23 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
 // are 8^3 force cells (LDS for phi + force of 14^3 and 3*10^3 values).
 template <typename R, int IZ, int BD>
 void launch_deposit_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
-                       const int64_t* off, R* mesh, float mass, PMOut* out, int testrho, hipStream_t s) {
+                       const int64_t* off, R* mesh, float mass, hipStream_t s) {
   const int nblk = (L.win_hi - L.win_lo + 1 + BD - 1) / BD;
   const int64_t grid = (int64_t)nblk * nblk * nblk * nbox;
   hipLaunchKernelGGL((k_deposit<R, IZ, BD>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
-                     (const XT<IZ>*)xp, rhoc, off, mesh, mass, out, testrho);
+                     (const XT<IZ>*)xp, rhoc, off, mesh, mass);
   CUBE_HIP(hipGetLastError());
 }
 
 template <typename R, int IZ>
 void launch_deposit(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
-                    const int64_t* off, R* mesh, float mass, PMOut* out, int testrho, hipStream_t s) {
-  if (std::is_same<R, float>::value && L.w > 1)
-    launch_deposit_bd<R, IZ, 16>(L, boxes, nbox, xp, rhoc, off, mesh, mass, out, testrho, s);
+                    const int64_t* off, R* mesh, float mass, hipStream_t s) {
+  // a block reads at most (BD/w + 2)^3 <= 216 coarse cells for w >= 4
+  if (L.w < 4) throw Error(1, "fine-level deposit needs >= 4 mesh cells per coarse cell");
+  if (std::is_same<R, float>::value)
+    launch_deposit_bd<R, IZ, 16>(L, boxes, nbox, xp, rhoc, off, mesh, mass, s);
   else
-    launch_deposit_bd<R, IZ, 8>(L, boxes, nbox, xp, rhoc, off, mesh, mass, out, testrho, s);
+    launch_deposit_bd<R, IZ, 8>(L, boxes, nbox, xp, rhoc, off, mesh, mass, s);
 }
 
 template <typename R, int IZ, int BD>
@@ -584,8 +738,13 @@ void launch_interp_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void
                       const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
   const int nblk = (L.nforce + BD - 1) / BD;
   const int64_t grid = (int64_t)nblk * nblk * nblk * nbox;
-  hipLaunchKernelGGL((k_interp<R, IZ, BD>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
-                     (const XT<IZ>*)xp, vp, rhoc, off, phi, a_mid, dt, out);
+  // a block reads at most (BD/w + 1)^3 coarse cells
+  if (L.w >= 4)
+    hipLaunchKernelGGL((k_interp<R, IZ, BD, (BD / 4 + 1) * (BD / 4 + 1) * (BD / 4 + 1)>), dim3((unsigned)grid),
+                       dim3(256), 0, s, L, boxes, (const XT<IZ>*)xp, vp, rhoc, off, phi, a_mid, dt, out);
+  else
+    hipLaunchKernelGGL((k_interp<R, IZ, BD, (BD + 1) * (BD + 1) * (BD + 1)>), dim3((unsigned)grid), dim3(256), 0,
+                       s, L, boxes, (const XT<IZ>*)xp, vp, rhoc, off, phi, a_mid, dt, out);
   CUBE_HIP(hipGetLastError());
 }
 
@@ -599,10 +758,34 @@ void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* x
 
 template <typename R>
 void pm_deposit(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
-                const int64_t* off, R* mesh, float mass, PMOut* out, hipStream_t s) {
-  const int testrho = L.periodic;  // only the coarse level reports testrho
-  if (izipx == 2) launch_deposit<R, 2>(L, boxes, nbox, xp, rhoc, off, mesh, mass, out, testrho, s);
-  else launch_deposit<R, 1>(L, boxes, nbox, xp, rhoc, off, mesh, mass, out, testrho, s);
+                const int64_t* off, R* mesh, float mass, hipStream_t s) {
+  if (izipx == 2) launch_deposit<R, 2>(L, boxes, nbox, xp, rhoc, off, mesh, mass, s);
+  else launch_deposit<R, 1>(L, boxes, nbox, xp, rhoc, off, mesh, mass, s);
+}
+
+void pm_level_scale(PMLevel& L) {
+  int e = 0;
+  const double m = std::frexp(L.gridsize, &e);
+  L.xdiv = (m == 0.5) ? 0 : 1;
+  L.xmul = (double)L.rcs / L.gridsize;
+}
+
+size_t pm1_stencil_bytes(const PMLevel& L) { return (size_t)27 * L.n * L.n * L.n * 8; }
+
+template <typename R>
+void pm1_deposit(int izipx, const PMLevel& L, const void* xp, const int32_t* rhoc, const int64_t* off,
+                 void* stencil, R* mesh, float mass, PMOut* out, hipStream_t s) {
+  if (L.w != 1 || !L.periodic || L.n % L.nt != 0) throw Error(1, "coarse deposit needs the periodic w=1 level");
+  const int64_t ncell = (int64_t)L.n * L.n * L.n;
+  const unsigned g1 = (unsigned)ceil_div(ncell, 2);
+  if (izipx == 2)
+    hipLaunchKernelGGL((k_pm1_stencil<R, 2>), dim3(g1), dim3(128), 0, s, L, (const int16_t*)xp, rhoc, off, stencil);
+  else
+    hipLaunchKernelGGL((k_pm1_stencil<R, 1>), dim3(g1), dim3(128), 0, s, L, (const int8_t*)xp, rhoc, off, stencil);
+  CUBE_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pm1_gather<R>, dim3((unsigned)ceil_div(ncell, 256)), dim3(256), 0, s, L,
+                     (const void*)stencil, mesh, mass, out);
+  CUBE_HIP(hipGetLastError());
 }
 
 template <typename R>
@@ -614,9 +797,13 @@ void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const 
 }
 
 template void pm_deposit<float>(int, const PMLevel&, const PMBox*, int, const void*, const int32_t*, const int64_t*,
-                                float*, float, PMOut*, hipStream_t);
+                                float*, float, hipStream_t);
 template void pm_deposit<double>(int, const PMLevel&, const PMBox*, int, const void*, const int32_t*, const int64_t*,
-                                 double*, float, PMOut*, hipStream_t);
+                                 double*, float, hipStream_t);
+template void pm1_deposit<float>(int, const PMLevel&, const void*, const int32_t*, const int64_t*, void*, float*, float,
+                                 PMOut*, hipStream_t);
+template void pm1_deposit<double>(int, const PMLevel&, const void*, const int32_t*, const int64_t*, void*, double*,
+                                  float, PMOut*, hipStream_t);
 template void pm_interp<float>(int, const PMLevel&, const PMBox*, int, const void*, float*, const int32_t*,
                                const int64_t*, const float*, float, float, PMOut*, hipStream_t);
 template void pm_interp<double>(int, const PMLevel&, const PMBox*, int, const void*, float*, const int32_t*,

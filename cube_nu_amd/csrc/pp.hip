@@ -1,9 +1,10 @@
 // pp.hip -- PP_Force (kick.f90:376-588) on the GPU.
 //
 // Binning: every PP cell (1/ratio_sf of a coarse cell) is nested in exactly
-// one coarse cell, so one thread per (sub-tile, coarse cell) owns the PP cells
-// it fills: counts and in-cell order are deterministic without atomics, and
-// the in-cell order is the reference's (coarse-cell particle order).
+// one coarse cell, so one wavefront per (sub-tile, coarse cell) owns the PP
+// cells it fills: counts and in-cell order are deterministic without
+// atomics, and the in-cell order is the reference's (coarse-cell particle
+// order).
 //
 // Force: one workgroup per brick of PB^3 physical PP cells; each thread owns
 // target particles and sums the pair force from the 27 neighbouring PP cells
@@ -36,47 +37,128 @@ __device__ inline bool pp_cell(const PPLevel& L, const PPBox& B, int ci, int cj,
   return in;
 }
 
-// One thread per (box, coarse cell of the sub-tile + 1-cell ring).
+// One wavefront per (box, coarse cell of the sub-tile + 1-cell ring), one
+// particle per lane.  A PP cell lies inside one coarse cell (idx =
+// floor(sf*(xvec - ifs*ntt)) + 1 with xvec < c), so the wavefront owns the
+// counts of the sf^3 PP cells of its coarse cell and keeps them in LDS: for
+// each distinct PP cell among 64 particles (one ballot per cell) the lanes
+// get their rank from a population count, so the in-cell order is the
+// reference's coarse-cell particle order.  FILL=0 writes the counts, FILL=1
+// writes the entries at start + rank.
 template <int IZ, int FILL>
-__global__ void k_pp_bin(PPLevel L, const PPBox* __restrict__ boxes, int nbox, const XT<IZ>* __restrict__ xp,
-                         const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off, int32_t* __restrict__ count,
-                         const int64_t* __restrict__ start, float4* __restrict__ ent) {
+__global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restrict__ boxes, int nbox,
+                                                const XT<IZ>* __restrict__ xp, const int32_t* __restrict__ rhoc,
+                                                const int64_t* __restrict__ off, int32_t* __restrict__ count,
+                                                const int64_t* __restrict__ start, float4* __restrict__ ent) {
+  __shared__ int lcnt[4][512];  // sf^3 <= 512 (ratio_sf <= 8)
   const int nr = L.ntt + 2;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nbox * nr * nr * nr) return;
-  const int ib = (int)(t / (nr * nr * nr));
-  const int q = (int)(t % (nr * nr * nr));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wv >= (int64_t)nbox * nr * nr * nr) return;
+  const int ib = (int)(wv / (nr * nr * nr));
+  const int q = (int)(wv % (nr * nr * nr));
   const PPBox B = boxes[ib];
-  const int ci = B.ifs[0] * L.ntt + (q % nr);  // nc1-1 + offset  (Fortran: nc1 = ifs*ntt+1)
-  const int cj = B.ifs[1] * L.ntt + (q / nr) % nr;
-  const int ck = B.ifs[2] * L.ntt + q / (nr * nr);
-  const int64_t cell = rc_index(ci, cj, ck, B.T, L.nte, L.ncb);
+  const int c[3] = {B.ifs[0] * L.ntt + (q % nr), B.ifs[1] * L.ntt + (q / nr) % nr,
+                    B.ifs[2] * L.ntt + q / (nr * nr)};  // nc1-1 + offset (Fortran nc1 = ifs*ntt+1)
+  const int64_t cell = rc_index(c[0], c[1], c[2], B.T, L.nte, L.ncb);
   const int cnt = rhoc[cell];
   const int64_t base = off[cell];
   const int m = L.npgrid + 2;
-  int32_t* cb = count + (int64_t)ib * m * m * m;
-  for (int l = 0; l < cnt; ++l) {
-    const int64_t ip = base + l;
-    float xf[3];
-    int idx[3];
-    if (!pp_cell<IZ>(L, B, ci, cj, ck, xp, ip, xf, idx)) continue;
-    const int64_t pc = ((int64_t)idx[2] * m + idx[1]) * m + idx[0];
-    if (FILL) {
-      const int64_t pos = start[(int64_t)ib * m * m * m + pc] + cb[pc];
-      ent[pos] = make_float4(xf[0], xf[1], xf[2], __uint_as_float((unsigned)ip));
+  const int sf = L.sf, sf3 = sf * sf * sf;
+  int lo[3];  // PP index of the coarse cell's first PP cell per axis
+  for (int d = 0; d < 3; ++d) lo[d] = sf * (c[d] - 1 - B.ifs[d] * L.ntt) + 1;
+  int* lc = lcnt[wave];
+  for (int t = lane; t < sf3; t += 64) lc[t] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int64_t cbase = (int64_t)ib * m * m * m;
+  for (int l0 = 0; l0 < cnt; l0 += 64) {
+    const int l = l0 + lane;
+    float xf[3] = {0, 0, 0};
+    int idx[3] = {0, 0, 0};
+    bool in = false;
+    if (l < cnt) in = pp_cell<IZ>(L, B, c[0], c[1], c[2], xp, base + l, xf, idx);
+    const int ls = in ? ((idx[2] - lo[2]) * sf + (idx[1] - lo[1])) * sf + (idx[0] - lo[0]) : -1;
+    bool todo = in;
+    while (true) {
+      const unsigned long long act = __ballot(todo);
+      if (act == 0) break;
+      const int leader = __ffsll((long long)act) - 1;
+      const int lls = __shfl(ls, leader, 64);
+      const unsigned long long mk = __ballot(todo && ls == lls);
+      const int prev = lc[lls];  // wave-uniform read
+      if (todo && ls == lls) {
+        if (FILL) {
+          const int64_t pc = cbase + ((int64_t)idx[2] * m + idx[1]) * m + idx[0];
+          const int64_t pos = start[pc] + prev + (int)__popcll(mk & ((1ull << lane) - 1ull));
+          ent[pos] = make_float4(xf[0], xf[1], xf[2], __uint_as_float((unsigned)(base + l)));
+        }
+        todo = false;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane == leader) lc[lls] = prev + (int)__popcll(mk);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    cb[pc] += 1;
+  }
+  if (!FILL) {
+    for (int t = lane; t < sf3; t += 64) {
+      const int n = lc[t];
+      if (n == 0) continue;
+      const int ix = lo[0] + t % sf, iy = lo[1] + (t / sf) % sf, iz = lo[2] + t / (sf * sf);
+      count[cbase + ((int64_t)iz * m + iy) * m + ix] = n;
+    }
   }
 }
 
-constexpr int PB = 4;  // physical PP cells per brick per dim
+constexpr int PB = 4;       // physical PP cells per brick per dim
+constexpr int kUnroll = 8;  // candidate loads in flight per lane
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FP32 pair law: F(r, app) - F(r, r_pp) with the
+// polynomials in Horner form (FMA) and both branches of F(r, r_pp) evaluated
+// without divergence; r < r_pp for every queued pair, so F(r, r_pp) is one of
+// the two polynomials.  F(r, app) = 1/r^2 except for r <= app (rare).
+__device__ inline float pair_law_fast(const PPLevel& L, float r2, float& rinv_out) {
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * rinv;
+  float fa = rinv * rinv;
+  const float ea = (2.0f * r) * L.inv_app;
+  if (ea <= 2.0f) {
+    const float e2 = ea * ea;
+    fa = ea > 1.0f
+             ? L.c_app * (12.0f * __builtin_amdgcn_rcpf(e2) +
+                          fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, 7.0f, -48.0f), 70.0f), 224.0f),
+                                                 -840.0f),
+                                        896.0f),
+                               -224.0f))
+             : L.c_app * ea * fmaf(e2, fmaf(ea, fmaf(ea, fmaf(ea, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+  }
+  const float ep = (2.0f * r) * L.inv_range;
+  const float e2 = ep * ep;
+  const float outer =
+      12.0f * __builtin_amdgcn_rcpf(e2) +
+      fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, -48.0f), 70.0f), 224.0f), -840.0f), 896.0f), -224.0f);
+  const float inner = ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+  const float fr = L.c_range * (ep > 1.0f ? outer : inner);
+  rinv_out = rinv;
+  return fa - fr;
+}
 
 // Pair force of candidate o on target me (kick.f90:523-530, 554-560):
 // f1 += (x2-x1)/r * (F(r,app) - F(r,r_pp)).
-// FP32 form: r^-1 from the hardware reciprocal square root and the F_ra
-// divisions replaced by products with precomputed reciprocals; the error
-// of a pair term stays at the level of FP32 rounding (the exact-division
-// form cost 1.7x the time).  FP64 form: the reference's operations in FP64.
+// FP32 form: r^-1 from the hardware reciprocal square root, the F_ra
+// divisions replaced by products with precomputed reciprocals and the
+// polynomials in Horner form (pair_law_fast); the error of a pair term stays
+// at the level of FP32 rounding.  FP64 form: the reference's operations in
+// FP64.
 template <typename R>
 __device__ inline void pair_force(const PPLevel& L, const float4 me, const float4 o, R f1[3]);
 
@@ -84,22 +166,12 @@ template <>
 __device__ inline void pair_force<float>(const PPLevel& L, const float4 me, const float4 o, float f1[3]) {
   const float rv0 = o.x - me.x, rv1 = o.y - me.y, rv2 = o.z - me.z;
   const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;
-  const float rinv = __builtin_amdgcn_rsqf(r2);
-  const float r = r2 * rinv;
-  auto F = [&](float inv_apm, float c) -> float {
-    const float ep = (2.0f * r) * inv_apm;
-    if (ep > 2.0f) return rinv * rinv;
-    const float e2 = ep * ep, e3 = e2 * ep, e4 = e2 * e2, e5 = e4 * ep, e6 = e3 * e3;
-    if (ep > 1.0f)
-      return c * ((((((12.0f * __builtin_amdgcn_rcpf(e2) - 224.0f) + 896.0f * ep) - 840.0f * e2) + 224.0f * e3) +
-                   70.0f * e4 - 48.0f * e5) +
-                  7.0f * e6);
-    return c * ((((224.0f * ep - 224.0f * e3) + 70.0f * e4) + 48.0f * e5) - 21.0f * e6);
-  };
-  const float s = rinv * (F(L.inv_app, L.c_app) - F(L.inv_range, L.c_range));
-  f1[0] = f1[0] + rv0 * s;
-  f1[1] = f1[1] + rv1 * s;
-  f1[2] = f1[2] + rv2 * s;
+  float rinv;
+  const float s = pair_law_fast(L, r2, rinv);
+  const float sc = rinv * s;
+  f1[0] = f1[0] + rv0 * sc;
+  f1[1] = f1[1] + rv1 * sc;
+  f1[2] = f1[2] + rv2 * sc;
 }
 
 template <>
@@ -116,14 +188,19 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4 me, con
 
 // One workgroup per brick of PB^3 physical PP cells; each thread owns target
 // particles and sums the pair forces of the 27 neighbouring PP cells in the
-// fixed (dz, dy, dx, p) order.  Neighbouring targets read the same
-// candidates, which therefore come from L1.
+// fixed (dz, dy, dx, p) order.  The (PB+2)^3 neighbourhood's counts and
+// starts are staged in LDS once; candidates are read kUnroll at a time so
+// that many loads are in flight per lane (they come from L1/L2: neighbouring targets
+// read the same candidates).
 template <typename R>
 __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __restrict__ count,
                                                   const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
                                                   PPOut* __restrict__ out) {
+  constexpr int NB = PB + 2, NB3 = NB * NB * NB;
   __shared__ int pre[PB * PB * PB + 1];
+  __shared__ int ncnt[NB3];
+  __shared__ int64_t nstart[NB3];
   __shared__ float red[4];
   __shared__ unsigned long long cnt_red[2];
   const int nbr = (L.npgrid + PB - 1) / PB;
@@ -137,12 +214,27 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __re
   const int m = L.npgrid + 2;
   const int64_t cbase = (int64_t)ib * m * m * m;
   const int c0[3] = {1 + bx * PB, 1 + by * PB, 1 + bz * PB};
+  // neighbourhood cells c0-1 .. c0+PB (clipped to the PP grid 0..npgrid+1)
+  for (int q = threadIdx.x; q < NB3; q += blockDim.x) {
+    const int x = c0[0] - 1 + q % NB, y = c0[1] - 1 + (q / NB) % NB, z = c0[2] - 1 + q / (NB * NB);
+    int c = 0;
+    int64_t st = 0;
+    if (x <= L.npgrid + 1 && y <= L.npgrid + 1 && z <= L.npgrid + 1) {
+      const int64_t nc = cbase + ((int64_t)z * m + y) * m + x;
+      c = count[nc];
+      st = start[nc];
+    }
+    ncnt[q] = c;
+    nstart[q] = st;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     int s = 0;
     for (int q = 0; q < PB * PB * PB; ++q) {
       pre[q] = s;
-      const int cx = c0[0] + q % PB, cy = c0[1] + (q / PB) % PB, cz = c0[2] + q / (PB * PB);
-      if (cx <= L.npgrid && cy <= L.npgrid && cz <= L.npgrid) s += count[cbase + ((int64_t)cz * m + cy) * m + cx];
+      const int lx = q % PB, ly = (q / PB) % PB, lz = q / (PB * PB);
+      if (c0[0] + lx <= L.npgrid && c0[1] + ly <= L.npgrid && c0[2] + lz <= L.npgrid)
+        s += ncnt[((lz + 1) * NB + ly + 1) * NB + lx + 1];
     }
     pre[PB * PB * PB] = s;
     cnt_red[0] = 0;
@@ -159,28 +251,32 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __re
       if (pre[mid] <= t) lo = mid;
       else hi = mid - 1;
     }
-    const int q = lo;
-    const int cx = c0[0] + q % PB, cy = c0[1] + (q / PB) % PB, cz = c0[2] + q / (PB * PB);
-    const int64_t tcell = cbase + ((int64_t)cz * m + cy) * m + cx;
-    const float4 me = ent[start[tcell] + (t - pre[q])];
+    const int lx = lo % PB, ly = (lo / PB) % PB, lz = lo / (PB * PB);
+    const int tq = ((lz + 1) * NB + ly + 1) * NB + lx + 1;
+    const float4 me = ent[nstart[tq] + (t - pre[lo])];
     R f1[3] = {0, 0, 0};
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
-          const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+          const int nx = c0[0] + lx + dx, ny = c0[1] + ly + dy, nz = c0[2] + lz + dz;
           const bool ghost = nx == 0 || ny == 0 || nz == 0 || nx == L.npgrid + 1 || ny == L.npgrid + 1 ||
                              nz == L.npgrid + 1;
-          const int64_t nc = cbase + ((int64_t)nz * m + ny) * m + nx;
-          const int64_t s0 = start[nc];
-          const int nn = count[nc];
-          for (int p = 0; p < nn; ++p) {
-            const float4 o = ent[s0 + p];
-            const float rv0 = o.x - me.x, rv1 = o.y - me.y, rv2 = o.z - me.z;
-            const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
-            if (0.0f < r2 && r2 < L.pp_range2) {
-              pair_force<R>(L, me, o, f1);
-              if (ghost) ++npg;
-              else ++npp;
+          const int nq = tq + (dz * NB + dy) * NB + dx;
+          const int64_t s0 = nstart[nq];
+          const int nn = ncnt[nq];
+          for (int p = 0; p < nn; p += kUnroll) {
+            float4 o[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) o[k] = ent[s0 + min(p + k, nn - 1)];
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k) {
+              const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
+              const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
+              if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
+                pair_force<R>(L, me, o[k], f1);
+                if (ghost) ++npg;
+                else ++npp;
+              }
             }
           }
         }
@@ -233,7 +329,7 @@ int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, 
   const int64_t ncell = (int64_t)nbox * m * m * m;
   zero_async(S.count, sizeof(int32_t) * ncell, s);
   const int nr = L.ntt + 2;
-  const int64_t nthr = (int64_t)nbox * nr * nr * nr;
+  const int64_t nthr = (int64_t)nbox * nr * nr * nr * 64;
   if (izipx == 2)
     hipLaunchKernelGGL((k_pp_bin<2, 0>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
                        (const int16_t*)xp, rhoc, off, S.count, nullptr, nullptr);
@@ -250,11 +346,8 @@ int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, 
 
 void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
                  const int64_t* off, PPScratch& S, hipStream_t s) {
-  const int m = L.npgrid + 2;
-  const int64_t ncell = (int64_t)nbox * m * m * m;
-  zero_async(S.count, sizeof(int32_t) * ncell, s);
   const int nr = L.ntt + 2;
-  const int64_t nthr = (int64_t)nbox * nr * nr * nr;
+  const int64_t nthr = (int64_t)nbox * nr * nr * nr * 64;
   if (izipx == 2)
     hipLaunchKernelGGL((k_pp_bin<2, 1>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
                        (const int16_t*)xp, rhoc, off, S.count, S.start, S.entries);

@@ -44,9 +44,10 @@ __device__ inline void dest_cell(const DriftParams& P, const XT<IZ>* xp, const f
 }
 
 __device__ inline void cell_of(const DriftParams& P, int64_t cglob, int* T, int ijk[3]) {
-  const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
-  *T = (int)(cglob / nct);
-  const int e = (int)(cglob % nct);
+  // extended cell counts stay below 2^31 (64 tiles of nte^3 cells)
+  const uint32_t nct = (uint32_t)(P.nte * P.nte * P.nte), cg = (uint32_t)cglob;
+  *T = (int)(cg / nct);
+  const int e = (int)(cg - (uint32_t)*T * nct);
   ijk[0] = e % P.nte + 1 - P.ncb;
   ijk[1] = (e / P.nte) % P.nte + 1 - P.ncb;
   ijk[2] = e / (P.nte * P.nte) + 1 - P.ncb;
@@ -261,33 +262,58 @@ __global__ void k_buffer_rhoc(Geo g, const int32_t* __restrict__ rp, int32_t* __
   if (phys) physcount[((T * g.nt + ijk[2] - 1) * g.nt + ijk[1] - 1) * g.nt + ijk[0] - 1] = v;
 }
 
+// One wavefront per extended cell: the cell's particles are a copy of its
+// periodic source cell's run in the physical layout.  Every lane loads its
+// share of the run (coordinates, velocities, PID) before any store.
+template <int IZ>
 __global__ void k_buffer_copy(Geo g, const int32_t* __restrict__ re, const int64_t* __restrict__ eoff,
-                              const int64_t* __restrict__ poff, const char* __restrict__ xp,
-                              const float* __restrict__ vp, const int64_t* __restrict__ pid, char* __restrict__ xo,
+                              const int64_t* __restrict__ poff, const XT<IZ>* __restrict__ xp,
+                              const float* __restrict__ vp, const int64_t* __restrict__ pid, XT<IZ>* __restrict__ xo,
                               float* __restrict__ vo, int64_t* __restrict__ po) {
   const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (c >= g.ncells_ext) return;
   const int cnt = re[c];
   if (cnt == 0) return;
-  const int64_t T = c / g.ncell_tile, e = c % g.ncell_tile;
-  const int64_t it[3] = {T % g.nnt, (T / g.nnt) % g.nnt, T / (g.nnt * g.nnt)};
-  const int64_t ijk[3] = {e % g.nte + 1 - g.ncb, (e / g.nte) % g.nte + 1 - g.ncb, e / (g.nte * g.nte) + 1 - g.ncb};
-  int64_t st[3], sc[3];
+  const int nte = (int)g.nte, nt = (int)g.nt, nc = (int)g.nc, nnt = (int)g.nnt, ncb = (int)g.ncb;
+  const uint32_t nct = (uint32_t)(nte * nte * nte), cu = (uint32_t)c;
+  const int T = (int)(cu / nct), e = (int)(cu - (uint32_t)T * nct);
+  const int it[3] = {T % nnt, (T / nnt) % nnt, T / (nnt * nnt)};
+  const int ijk[3] = {e % nte + 1 - ncb, (e / nte) % nte + 1 - ncb, e / (nte * nte) + 1 - ncb};
+  int st[3], sc[3];
   for (int d = 0; d < 3; ++d) {
-    int64_t gc = it[d] * g.nt + ijk[d] - 1;
-    gc = ((gc % g.nc) + g.nc) % g.nc;
-    st[d] = gc / g.nt;
-    sc[d] = gc % g.nt + 1;
+    int gc = it[d] * nt + ijk[d] - 1;  // in [-ncb, nc + ncb)
+    gc = gc < 0 ? gc + nc : (gc >= nc ? gc - nc : gc);
+    st[d] = gc / nt;
+    sc[d] = gc - st[d] * nt + 1;
   }
-  const int64_t Ts = (st[2] * g.nnt + st[1]) * g.nnt + st[0];
-  const int64_t src = poff[((Ts * g.nt + sc[2] - 1) * g.nt + sc[1] - 1) * g.nt + sc[0] - 1];
+  const int Ts = (st[2] * nnt + st[1]) * nnt + st[0];
+  const int64_t src = poff[(((int64_t)Ts * nt + sc[2] - 1) * nt + sc[1] - 1) * nt + sc[0] - 1];
   const int64_t dst = eoff[c];
-  const int esz = (int)g.izipx;
-  for (int q = lane; q < 3 * esz * cnt; q += 64) xo[3 * esz * dst + q] = xp[3 * esz * src + q];
-  for (int q = lane; q < 3 * cnt; q += 64) vo[3 * dst + q] = vp[3 * src + q];
-  if (pid)
-    for (int q = lane; q < cnt; q += 64) po[dst + q] = pid[src + q];
+  for (int b = 0; b < cnt; b += 64) {
+    const int n = min(64, cnt - b);
+    XT<IZ> xv[3];
+    float vv[3];
+    int64_t pv = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int q = lane + 64 * k;
+      if (q < 3 * n) {
+        xv[k] = xp[3 * (src + b) + q];
+        vv[k] = vp[3 * (src + b) + q];
+      }
+    }
+    if (pid && lane < n) pv = pid[src + b + lane];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int q = lane + 64 * k;
+      if (q < 3 * n) {
+        xo[3 * (dst + b) + q] = xv[k];
+        vo[3 * (dst + b) + q] = vv[k];
+      }
+    }
+    if (pid && lane < n) po[dst + b + lane] = pv;
+  }
 }
 
 }  // namespace
@@ -360,8 +386,12 @@ void buffer_copy(const Geo& g, const int32_t* rhoc_ext, const int64_t* ext_off, 
                  const void* xp, const float* vp, const int64_t* pid, void* xp_out, float* vp_out, int64_t* pid_out,
                  hipStream_t s) {
   const int64_t nthr = g.ncells_ext * 64;
-  hipLaunchKernelGGL(k_buffer_copy, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, g, rhoc_ext, ext_off,
-                     phys_off, (const char*)xp, vp, pid, (char*)xp_out, vp_out, pid_out);
+  if (g.izipx == 2)
+    hipLaunchKernelGGL(k_buffer_copy<2>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, g, rhoc_ext, ext_off,
+                       phys_off, (const int16_t*)xp, vp, pid, (int16_t*)xp_out, vp_out, pid_out);
+  else
+    hipLaunchKernelGGL(k_buffer_copy<1>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, g, rhoc_ext, ext_off,
+                       phys_off, (const int8_t*)xp, vp, pid, (int8_t*)xp_out, vp_out, pid_out);
   CUBE_HIP(hipGetLastError());
 }
 

@@ -298,8 +298,9 @@ __global__ void __launch_bounds__(NT) k_plane_inv(const CP<R>* __restrict__ in, 
 template <typename R, int N>
 bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t nbox, R div3, R div1,
                hipStream_t s, KTimer* tm, const std::string& t) {
-  // large planes: more threads so each holds fewer butterflies in registers
-  constexpr int NT = N >= 128 ? 1024 : 256;
+  // threads per plane / slice (measured at n = 80 and 160 on MI355X)
+  constexpr int NT = N >= 128 ? 1024 : (N >= 80 ? 512 : 256);
+  constexpr int NTZ = N >= 80 ? 1024 : 256;
   const double Gb = (double)N * N * N * sizeof(R) * nbox;
   const double Kb = (double)N * N * (N / 2 + 1) * sizeof(CP<R>) * nbox;
   const double Gk = (double)N * N * (N / 2 + 1) * sizeof(float);
@@ -310,7 +311,7 @@ bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t
   }
   {
     KScope k(tm, (t + "fft_z_green").c_str(), 2 * Kb + Gk);
-    hipLaunchKernelGGL((k_zslice<R, N, NT>), dim3((unsigned)(nbox * N)), dim3(NT), 0, s, work, G, c.tw);
+    hipLaunchKernelGGL((k_zslice<R, N, NTZ>), dim3((unsigned)(nbox * N)), dim3(NTZ), 0, s, work, G, c.tw);
     CUBE_HIP(hipGetLastError());
   }
   {

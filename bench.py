@@ -68,7 +68,23 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
     return {"seconds": t, "n": nloc, "report": rep, "dt": dt}
 
 
-def roofline(rep, steps):
+def measured_traffic(kernel, ng):
+    """HBM bytes per launch of `kernel` at ng^3 from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE runs, scaled by a copy-kernel calibration), or
+    None when no summary covers this kernel and size."""
+    import glob
+    import re
+
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md")):
+        m = re.search(r"machine-readable: (\{.*\})", open(path).read())
+        if m:
+            d = json.loads(m.group(1))
+            if d.get("kernel") == kernel and d.get("ng") == ng:
+                return d["traffic_bytes"]
+    return None
+
+
+def roofline(rep, steps, ng):
     kern = {k: v for k, v in rep.items() if v["ms"] > 0 and k != "green"}
     if not kern:
         return None, {}
@@ -81,11 +97,12 @@ def roofline(rep, steps):
         pairs = rep.get("pp_pairs_last", {}).get("calls", 0)
         ach = 40.0 * pairs / (v["ms"] / steps * 1e-3) / 1e12
         r = {"kernel": name, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-             "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "avg_launch_ms": avg_ms}
+             "frac": ach / FP32_PEAK_TFLOPS, "traffic": measured_traffic(name, ng), "avg_launch_ms": avg_ms,
+             "pairs_per_launch": pairs}
     else:
         ach = (v["bytes"] / v["calls"]) / (avg_ms * 1e-3) / 1e9
         r = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+             "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(name, ng), "avg_launch_ms": avg_ms,
              "bytes_per_launch": v["bytes"] / v["calls"]}
     return r, phases
 
@@ -156,7 +173,7 @@ def main():
         sys.exit(2)
     r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box)
     value = r32["n"] * args.steps / r32["seconds"]
-    roof, phases = roofline(r32["report"], args.steps)
+    roof, phases = roofline(r32["report"], args.steps, args.ng)
     fp64 = fp64_plain = None
     if args.fp64_steps > 0:
         def form(per_tile):

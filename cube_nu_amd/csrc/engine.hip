@@ -9,6 +9,7 @@
 #include <memory>
 
 #include "../../include/cube_p3m.h"
+#include "cic.h"
 #include "common.h"
 #include "drift.h"
 #include "fft.h"
@@ -910,6 +911,27 @@ int cube_p3m_make_ic(cube_p3m_ctx* ctx, const cube_ic_params* ic, float* vz_max,
     ctx->buffer(&n);
     if (vz_max) *vz_max = r.vz_max;
     if (nplocal) *nplocal = r.nplocal;
+  });
+}
+
+int cube_p3m_cicpower(cube_p3m_ctx* ctx, int32_t nw, float* xi, int32_t nbin, float* delta, double* mean) {
+  return guarded(ctx, [&] {
+    if (!xi) throw Error(CUBE_ERR_ARG, "null argument");
+    if (ctx->layout == 0) throw Error(CUBE_ERR_STATE, "no resident particles");
+    const Geo& g = ctx->g;
+    ctx->ensure_offsets();
+    // physical particle count (n_particle of auto_power)
+    std::vector<int32_t> rc((size_t)g.ncells_ext);
+    CUBE_HIP(hipMemcpyAsync(rc.data(), ctx->rhocc(), sizeof(int32_t) * rc.size(), hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+    int64_t npart = 0;
+    for (int64_t T = 0; T < g.ntiles; ++T)
+      for (int64_t k = 1; k <= g.nt; ++k)
+        for (int64_t j = 1; j <= g.nt; ++j)
+          for (int64_t i = 1; i <= g.nt; ++i) npart += rc[rc_index(i, j, k, T, g.nte, g.ncb)];
+    if (npart == 0) throw Error(CUBE_ERR_STATE, "no physical particles");
+    KScope k(&ctx->tm, "cicpower", 0);
+    cic_power(g, nw, ctx->cfg.box, ctx->xpc(), ctx->rhocc(), ctx->off.p, npart, xi, nbin, delta, mean, ctx->s);
   });
 }
 

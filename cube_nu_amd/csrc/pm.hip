@@ -39,7 +39,6 @@ __device__ inline int floor_div(int a, int b) {
 template <int IZ>
 using XT = typename XP<IZ>::T;
 
-constexpr int kMaxCells = 1000;  // coarse cells one block reads: (8+2)^3 on the coarse level
 
 // Exclusive block scan of cnt[0..n) into pre[0..n] (pre[n] = total), n <= 1024.
 __device__ inline void block_scan(const int* cnt, int* pre, int n, int* wsum) {

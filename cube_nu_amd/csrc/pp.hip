@@ -116,11 +116,6 @@ __global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restri
 constexpr int PB = 4;       // physical PP cells per brick per dim
 constexpr int kUnroll = 8;  // candidate loads in flight per lane
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // FP32 pair law: F(r, app) - F(r, r_pp) with the
 // polynomials in Horner form (FMA) and both branches of F(r, r_pp) evaluated

@@ -99,8 +99,8 @@ ABI_FUNCTIONS = (
     "cube_p3m_create", "cube_p3m_destroy", "cube_p3m_last_error", "cube_p3m_set_tf", "cube_p3m_get_tf",
     "cube_p3m_particle_mesh", "cube_p3m_kick", "cube_p3m_drift", "cube_p3m_buffer", "cube_p3m_state_upload",
     "cube_p3m_state_download", "cube_p3m_kick_resident", "cube_p3m_drift_resident", "cube_p3m_buffer_resident",
-    "cube_p3m_step_resident", "cube_p3m_make_ic", "cube_p3m_synchronize", "cube_p3m_timing_enable",
-    "cube_p3m_timing_reset", "cube_p3m_timing_report", "cube_p3m_version",
+    "cube_p3m_step_resident", "cube_p3m_make_ic", "cube_p3m_cicpower", "cube_p3m_synchronize",
+    "cube_p3m_timing_enable", "cube_p3m_timing_reset", "cube_p3m_timing_report", "cube_p3m_version",
 )
 
 _lib = None
@@ -136,6 +136,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cube_p3m_step_resident.argtypes = [P, C.POINTER(DriftIn), C.POINTER(KickIn), C.POINTER(DriftOut),
                                            C.POINTER(KickOut)]
     lib.cube_p3m_make_ic.argtypes = [P, C.POINTER(ICParams), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+    lib.cube_p3m_cicpower.argtypes = [P, C.c_int32, vp_, C.c_int32, vp_, C.POINTER(C.c_double)]
     lib.cube_p3m_synchronize.argtypes = [P]
     lib.cube_p3m_timing_enable.argtypes = [P, C.c_int32]
     lib.cube_p3m_timing_reset.argtypes = [P]
@@ -375,6 +376,19 @@ class P3M:
         n = C.c_int64()
         self._check(self.lib.cube_p3m_make_ic(self.ctx, C.byref(p), C.byref(vz), C.byref(n)))
         return {"vz_max": vz.value, "nplocal": n.value}
+
+    def cicpower(self, nw: int, want_delta: bool = False):
+        """CIC P(k) of the resident physical particles on an nw^3 mesh
+        (utilities/cicpower.f90 + powerspectrum.f90 auto_power).  Returns
+        (xi (nbin+1, 10) float32, mean, delta (nw,nw,nw) or None)."""
+        nbin = int(round((nw // 2) * float(np.sqrt(np.float32(3.0)))))
+        xi = np.zeros((nbin + 1, 10), np.float32)
+        delta = np.zeros((nw, nw, nw), np.float32) if want_delta else None
+        mean = C.c_double()
+        self._check(self.lib.cube_p3m_cicpower(self.ctx, nw, xi.ctypes.data_as(C.c_void_p), nbin,
+                                               None if delta is None else delta.ctypes.data_as(C.c_void_p),
+                                               C.byref(mean)))
+        return xi, mean.value, delta
 
     def synchronize(self):
         self._check(self.lib.cube_p3m_synchronize(self.ctx))

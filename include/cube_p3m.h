@@ -185,6 +185,19 @@ int cube_p3m_step_resident(cube_p3m_ctx* ctx, const cube_drift_in* din,
 int cube_p3m_make_ic(cube_p3m_ctx* ctx, const cube_ic_params* ic,
                      float* vz_max, int64_t* nplocal);
 
+/* ---- analysis on the resident state ------------------------------------- */
+
+/* CIC power spectrum of the physical particles on an nw^3 periodic mesh:
+ * utilities/cicpower.f90:67-245 (deposit, delta = rho/mean - 1) with
+ * utilities/powerspectrum.f90:140-259 (auto_power, CIC window, shot noise,
+ * pk_correction).  xi is the reference's xi(10, 0:nbin), column-major,
+ * nbin = nint(nw/2*sqrt(3.)) (powerspectrum.f90:14); columns 1-5 are filled:
+ * count, k (h/Mpc), raw power, shot-noise subtracted, alias corrected
+ * ((Mpc/h)^3, box from the configuration).  delta (nw^3, x fastest) and mean
+ * (FP64 mean CIC mass per cell) may be NULL. */
+int cube_p3m_cicpower(cube_p3m_ctx* ctx, int32_t nw, float* xi, int32_t nbin,
+                      float* delta, double* mean);
+
 /* ---- measurement -------------------------------------------------------- */
 int cube_p3m_synchronize(cube_p3m_ctx* ctx);
 /* Enable HIP-event timing of every kernel launch on the context stream. */

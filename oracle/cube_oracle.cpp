@@ -1161,6 +1161,175 @@ int orc_buffer(const OParams* P, const void* xp, const float* vp, const int64_t*
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// CIC power spectrum of the physical particles (utilities/cicpower.f90:67-179
+// with utilities/powerspectrum.f90:140-259 auto_power and pk_correction),
+// nn = 1.  Per-mode quantities in the reference's precision (FP32 positions,
+// weights, kr, amp, C1k); the CIC mesh accumulates in FP32 in the reference's
+// loop order; bin sums are FP64 (the reference's FP32 bin sums are
+// order-dependent).  xi is (10, 0:nbin) column-major: xi[b*10 + c].
+static double interp1_ref(const std::vector<double>& x, const std::vector<double>& y, int nbin, double xq) {
+  // powerspectrum.f90:261-275 on xdata(1:nbin) = bins 0..nbin-1
+  int i1 = 1, i2 = nbin;
+  while (i2 - i1 > 1) {
+    int im = (i1 + i2) / 2;
+    if (xq > x[im - 1]) i1 = im;
+    else i2 = im;
+  }
+  return y[i1 - 1] + (xq - x[i1 - 1]) / (x[i2 - 1] - x[i1 - 1]) * (y[i2 - 1] - y[i1 - 1]);
+}
+
+int orc_cicpower(const OParams* P, const void* xp, const int32_t* rhoc_phys, int32_t nw, float box, float* delta,
+                 double* mean_out, float* xi_out, int32_t nbin) {
+  Geo g(*P);
+  const int64_t nt = g.nt, nnt = g.nnt, nc = g.nc;
+  const int64_t m = nw + 2;
+  std::vector<float> grid((size_t)m * m * m, 0.0f);
+  auto G = [&](int64_t i, int64_t j, int64_t k) -> float& { return grid[((size_t)k * m + j) * m + i]; };  // 0..nw+1
+  int64_t nlast = 0;
+  for (int64_t itz = 1; itz <= nnt; ++itz)
+    for (int64_t ity = 1; ity <= nnt; ++ity)
+      for (int64_t itx = 1; itx <= nnt; ++itx) {
+        const int64_t T = g.tile_lin(itx, ity, itz);
+        const int64_t it[3] = {itx, ity, itz};
+        for (int64_t k = 1; k <= nt; ++k)
+          for (int64_t j = 1; j <= nt; ++j)
+            for (int64_t i = 1; i <= nt; ++i) {
+              const int64_t np = rhoc_phys[g.rc(i, j, k, T)];
+              const int64_t ijk[3] = {i, j, k};
+              for (int64_t l = 1; l <= np; ++l) {
+                const int64_t ip = nlast + l - 1;
+                float pos[3], dx1[3], dx2[3];
+                int64_t i1[3], i2[3];
+                for (int d = 0; d < 3; ++d) {
+                  pos[d] = (float)((double)(nt * (it[d] - 1) + (ijk[d] - 1)) + decode(xp, ip, d, g));
+                  pos[d] = (pos[d] * (float)nw) / (float)nc - 0.5f;  // cicpower.f90:79
+                  i1[d] = (int64_t)std::floor(pos[d]) + 1;
+                  i2[d] = i1[d] + 1;
+                  dx1[d] = (float)i1[d] - pos[d];
+                  dx2[d] = 1.0f - dx1[d];
+                }
+                G(i1[0], i1[1], i1[2]) += (dx1[0] * dx1[1]) * dx1[2];
+                G(i2[0], i1[1], i1[2]) += (dx2[0] * dx1[1]) * dx1[2];
+                G(i1[0], i2[1], i1[2]) += (dx1[0] * dx2[1]) * dx1[2];
+                G(i1[0], i1[1], i2[2]) += (dx1[0] * dx1[1]) * dx2[2];
+                G(i1[0], i2[1], i2[2]) += (dx1[0] * dx2[1]) * dx2[2];
+                G(i2[0], i1[1], i2[2]) += (dx2[0] * dx1[1]) * dx2[2];
+                G(i2[0], i2[1], i1[2]) += (dx2[0] * dx2[1]) * dx1[2];
+                G(i2[0], i2[1], i2[2]) += (dx2[0] * dx2[1]) * dx2[2];
+              }
+              nlast += np;
+            }
+      }
+  // periodic halo, x then y then z (cicpower.f90:108-156 at nn = 1)
+  for (int64_t a = 0; a < m; ++a)
+    for (int64_t b = 0; b < m; ++b) { G(1, a, b) += G(nw + 1, a, b); G(nw, a, b) += G(0, a, b); }
+  for (int64_t a = 0; a < m; ++a)
+    for (int64_t b = 0; b < m; ++b) { G(a, 1, b) += G(a, nw + 1, b); G(a, nw, b) += G(a, 0, b); }
+  for (int64_t a = 0; a < m; ++a)
+    for (int64_t b = 0; b < m; ++b) { G(a, b, 1) += G(a, b, nw + 1); G(a, b, nw) += G(a, b, 0); }
+  const int64_t n3 = (int64_t)nw * nw * nw;
+  std::vector<float> rc((size_t)n3);
+  double rho8 = 0;
+  for (int64_t k = 1; k <= nw; ++k)
+    for (int64_t j = 1; j <= nw; ++j)
+      for (int64_t i = 1; i <= nw; ++i) {
+        float v = G(i, j, k);
+        rc[((size_t)(k - 1) * nw + (j - 1)) * nw + (i - 1)] = v;
+        rho8 += (double)v;
+      }
+  const double mean = rho8 / (double)nw / (double)nw / (double)nw;
+  for (auto& v : rc) v = (float)((double)v / mean - 1.0);  // cicpower.f90:176-179
+  if (delta) std::memcpy(delta, rc.data(), sizeof(float) * n3);
+  if (mean_out) *mean_out = mean;
+  // auto_power (powerspectrum.f90:140-200)
+  const int64_t nh = nw / 2 + 1, nyq = nw / 2;
+  std::vector<cd> ck((size_t)nh * nw * nw);
+  r2c3d(nw, rc.data(), ck.data());
+  std::vector<double> x1(nbin + 1, 0), x2(nbin + 1, 0), x3(nbin + 1, 0), x4(nbin + 1, 0);
+  const float fnw = (float)nw;
+  for (int64_t kg = 1; kg <= nw; ++kg)
+    for (int64_t jg = 1; jg <= nw; ++jg)
+      for (int64_t ig = 1; ig <= nyq + 1; ++ig) {
+        const int64_t kk[3] = {((ig + nyq - 1) % nw) - nyq, ((jg + nyq - 1) % nw) - nyq, ((kg + nyq - 1) % nw) - nyq};
+        if (ig == 1 && jg == 1 && kg == 1) continue;
+        if ((ig == 1 || ig == nyq + 1) && jg > nyq + 1) continue;
+        if ((ig == 1 || ig == nyq + 1) && (jg == 1 || jg == nyq + 1) && kg > nyq + 1) continue;
+        const float kx[3] = {(float)kk[0], (float)kk[1], (float)kk[2]};
+        const float kr = std::sqrt((kx[0] * kx[0] + kx[1] * kx[1]) + kx[2] * kx[2]);
+        const int64_t ib = (int64_t)std::lround(kr);
+        if (ib > nbin) continue;
+        const cd c = ck[((size_t)(kg - 1) * nw + (jg - 1)) * nh + (ig - 1)];
+        const float re = (((float)c.real() / fnw) / fnw) / fnw, im = (((float)c.imag() / fnw) / fnw) / fnw;
+        const float amp = re * re + im * im;
+        float c1[3];
+        for (int d = 0; d < 3; ++d) {
+          const double sn = std::sin(PI * (double)kx[d] / (double)nw);
+          c1[d] = (float)(1.0 - (double)(2.0f / 3.0f) * (sn * sn));
+        }
+        int64_t npart = nlast;
+        const float Dk = ((c1[0] * c1[1]) * c1[2]) / (float)npart;
+        x1[ib] += 1;
+        x2[ib] += kr;
+        x3[ib] += amp;
+        x4[ib] += (double)(amp - Dk);
+      }
+  std::vector<double> x5(nbin + 1);
+  for (int b = 0; b <= nbin; ++b) {
+    x2[b] /= x1[b]; x3[b] /= x1[b]; x4[b] /= x1[b]; x5[b] = x4[b];
+  }
+  // pk_correction (powerspectrum.f90:213-259), applied while alpha is a number
+  for (int iter = 0; iter < 4; ++iter) {
+    const double alpha = (std::log(interp1_ref(x2, x5, nbin, (double)(float)nyq)) -
+                          std::log(interp1_ref(x2, x5, nbin, (double)(float)nyq / 2))) / std::log(2.0);
+    if (std::isnan(alpha)) break;
+    std::vector<double> c0(nbin + 1, 0), c2(nbin + 1, 0);
+#pragma omp parallel
+    {
+      std::vector<double> l0(nbin + 1, 0), l2(nbin + 1, 0);
+#pragma omp for schedule(dynamic)
+      for (int64_t i = 1; i <= nyq + 1; ++i)
+        for (int64_t j = 1; j <= i; ++j)
+          for (int64_t k = 1; k <= j; ++k) {
+            const double kv[3] = {(double)(i - 1), (double)(j - 1), (double)(k - 1)};
+            const double kmag = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+            const int64_t ib = (int64_t)std::lround(kmag);
+            if (ib < 1 || ib > nbin) continue;
+            double w = 0;
+            for (int in = -3; in <= 3; ++in)
+              for (int jn = -3; jn <= 3; ++jn)
+                for (int kn = -3; kn <= 3; ++kn) {
+                  const double kvn[3] = {kv[0] + in * (double)nw, kv[1] + jn * (double)nw, kv[2] + kn * (double)nw};
+                  const double kmn = std::sqrt(kvn[0] * kvn[0] + kvn[1] * kvn[1] + kvn[2] * kvn[2]);
+                  double pr = 1;
+                  for (int d = 0; d < 3; ++d) {
+                    const double ks = PI * kvn[d] / (double)nw;
+                    pr *= (ks == 0) ? 1.0 : std::sin(ks) / ks;
+                  }
+                  const double p2 = pr * pr;
+                  w += (p2 * p2) * std::pow(kmn, alpha);  // sinc^(2p), p = 2 (CIC)
+                }
+            l0[ib] += 1;
+            l2[ib] += w / std::pow(kmag, alpha);
+          }
+#pragma omp critical(pkc)
+      for (int b = 0; b <= nbin; ++b) { c0[b] += l0[b]; c2[b] += l2[b]; }
+    }
+    for (int b = 1; b <= nbin; ++b) x5[b] = x4[b] / (c2[b] / c0[b]);
+  }
+  const double kfac = 2 * PI / (double)box, b3 = (double)box * box * box;
+  for (int b = 0; b <= nbin; ++b) {
+    float* o = xi_out + (size_t)b * 10;
+    for (int c = 0; c < 10; ++c) o[c] = 0;
+    o[0] = (float)x1[b];
+    o[1] = (float)(x2[b] * kfac);
+    o[2] = (float)(x3[b] * b3);
+    o[3] = (float)(x4[b] * b3);
+    o[4] = (float)(x5[b] * b3);
+  }
+  return 0;
+}
+
 int orc_num_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

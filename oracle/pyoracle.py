@@ -232,5 +232,17 @@ def buffer(P, xp, vp, pid, rhoc_phys, cap):
     return xo[:k], vo[:k], po[:k], ro
 
 
+def cicpower(P, xp, rhoc_phys, nw, box, want_delta=False):
+    """CIC P(k) of a physical layout: (xi (nbin+1, 10), mean, delta or None)."""
+    nyq = nw // 2
+    nbin = int(round(nyq * float(np.sqrt(np.float32(3.0)))))
+    xi = np.zeros((nbin + 1, 10), np.float32)
+    delta = np.zeros((nw, nw, nw), np.float32) if want_delta else None
+    mean = C.c_double()
+    lib().orc_cicpower(C.byref(P), _p(np.ascontiguousarray(xp)), _p(rhoc_phys), C.c_int32(nw), C.c_float(box),
+                       _p(delta), C.byref(mean), _p(xi), C.c_int32(nbin))
+    return xi, mean.value, delta
+
+
 def num_threads():
     return lib().orc_num_threads()

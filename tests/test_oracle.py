@@ -420,3 +420,102 @@ def test_kick_step_criteria_closed_forms():
     phys = physical_mask(geo, rhoc)
     assert out["vz_max"] == np.abs(v[phys, 2]).max()  # post-PP max|v_z| (kick.f90:315)
     assert abs(out["dt_vmax"] - 0.9 * 14 / np.abs(v[phys]).max()) <= 1e-6 * out["dt_vmax"]
+
+
+# -------------------------------------------------------------- CIC power
+def _cic_numpy(geo, xp, rhoc, nw):
+    """CIC deposit of cicpower.f90:67-156 restated with numpy (FP32 positions
+    and weights, periodic wrap), mesh summed in FP64."""
+    nt, nnt, b, nte = geo.nt, geo.nnt, geo.ncb, geo.nte
+    ph = rhoc.reshape(nnt, nnt, nnt, nte, nte, nte)[..., b:b + nt, b:b + nt, b:b + nt]
+    cells = []
+    for tz in range(nnt):
+        for ty in range(nnt):
+            for tx in range(nnt):
+                kk, jj, ii = np.meshgrid(np.arange(nt) + tz * nt, np.arange(nt) + ty * nt, np.arange(nt) + tx * nt,
+                                         indexing="ij")
+                cells.append(np.repeat(np.stack([ii, jj, kk], -1).reshape(-1, 3), ph[tz, ty, tx].reshape(-1), 0))
+    gc = np.concatenate(cells)
+    pos = (gc + orc.decode(xp, geo.izipx)).astype(np.float32)
+    pos = ((pos * np.float32(nw)) / np.float32(geo.nc) - np.float32(0.5)).astype(np.float32)
+    f = np.floor(pos).astype(np.int64) + 1
+    dx1 = (f.astype(np.float32) - pos).astype(np.float32)
+    dx2 = (np.float32(1) - dx1).astype(np.float32)
+    grid = np.zeros((nw, nw, nw), np.float64)
+    for sx in (0, 1):
+        for sy in (0, 1):
+            for sz in (0, 1):
+                w = ((dx2[:, 0] if sx else dx1[:, 0]) * (dx2[:, 1] if sy else dx1[:, 1])).astype(np.float32)
+                w = (w * (dx2[:, 2] if sz else dx1[:, 2])).astype(np.float32)
+                ix = (f[:, 0] + sx - 1) % nw
+                iy = (f[:, 1] + sy - 1) % nw
+                iz = (f[:, 2] + sz - 1) % nw
+                np.add.at(grid, (iz, iy, ix), w.astype(np.float64))
+    return grid, gc.shape[0]
+
+
+def test_cic_density_matches_numpy_and_conserves_mass():
+    geo = Geometry(64)
+    xp, vp, pid, rhoc = physical_set(geo, seed=12, mean=10.0)
+    nw = 32
+    xi, mean, delta = orc.cicpower(oparams(geo), xp, rhoc, nw, 75.0, want_delta=True)
+    grid, n = _cic_numpy(geo, xp, rhoc, nw)
+    assert abs(mean * nw ** 3 - n) <= 1e-5 * n  # every particle deposits unit mass
+    ref = grid / (grid.sum() / nw ** 3) - 1
+    assert np.abs(delta - ref).max() <= 2e-5 * np.abs(ref).max()
+
+
+def test_cic_auto_power_matches_numpy_binning():
+    """Columns 1-4 of auto_power restated with numpy.fft and numpy binning
+    from the oracle's own delta field (powerspectrum.f90:158-199)."""
+    geo = Geometry(64)
+    xp, vp, pid, rhoc = physical_set(geo, seed=13, mean=20.0)
+    nw, box = 32, 75.0
+    xi, mean, delta = orc.cicpower(oparams(geo), xp, rhoc, nw, box, want_delta=True)
+    n = int(rhoc.sum())
+    ck = np.fft.rfftn(delta.astype(np.float64)) / nw ** 3  # [z][y][kx]
+    nyq = nw // 2
+    kf = lambda m: ((m + nyq) % nw) - nyq
+    kz, ky, kx = np.meshgrid(kf(np.arange(nw)), kf(np.arange(nw)), np.arange(nyq + 1), indexing="ij")
+    ig, jg, kg = kx + 1, np.arange(nw)[None, :, None] + 1 + 0 * kx, np.arange(nw)[:, None, None] + 1 + 0 * kx
+    keep = ~((ig == 1) & (jg == 1) & (kg == 1))
+    keep &= ~(((ig == 1) | (ig == nyq + 1)) & (jg > nyq + 1))
+    keep &= ~(((ig == 1) | (ig == nyq + 1)) & ((jg == 1) | (jg == nyq + 1)) & (kg > nyq + 1))
+    kr = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    ib = np.floor(kr + 0.5).astype(np.int64)
+    amp = np.abs(ck) ** 2
+    c1 = lambda k: 1 - (2.0 / 3.0) * np.sin(np.pi * k / nw) ** 2
+    dk = c1(kx) * c1(ky) * c1(kz) / n
+    nb = xi.shape[0]
+    cnt = np.bincount(ib[keep], minlength=nb)[:nb]
+    assert np.array_equal(xi[:, 0], cnt.astype(np.float32))
+    sel = cnt > 0
+    k_ref = np.bincount(ib[keep], kr[keep], nb)[:nb][sel] / cnt[sel] * 2 * np.pi / box
+    p3 = np.bincount(ib[keep], amp[keep], nb)[:nb][sel] / cnt[sel] * box ** 3
+    p4 = np.bincount(ib[keep], (amp - dk)[keep], nb)[:nb][sel] / cnt[sel] * box ** 3
+    assert np.allclose(xi[sel, 1], k_ref, rtol=1e-6)
+    assert np.allclose(xi[sel, 2], p3, rtol=2e-5)
+    assert np.abs(xi[sel, 3] - p4).max() <= 2e-5 * np.abs(p3).max()
+
+
+def test_cic_shot_noise_of_a_poisson_sample():
+    """Unclustered particles: <|delta_k|^2> = C_CIC(k)/N, the Dk the reference
+    subtracts, so the raw power equals box^3 C/N on average and the
+    shot-noise-subtracted power is zero within the sampling error."""
+    geo = Geometry(64)
+    rng = np.random.default_rng(8)
+    n = 120000
+    pos = rng.uniform(0, geo.nc, (n, 3))
+    from inputs import explicit_set
+
+    xp, vp, pid, rhoc = explicit_set(geo, pos)
+    nw, box = 32, 75.0
+    xi, mean, _ = orc.cicpower(oparams(geo), xp, rhoc, nw, box)
+    nyq = nw // 2
+    b = slice(1, nyq)  # up to Nyquist
+    w = xi[b, 0]
+    shot = box ** 3 / n
+    ratio = np.sum(w * xi[b, 2]) / np.sum(w) / shot
+    c_mean = np.sum(w * (xi[b, 2] - xi[b, 3])) / np.sum(w) / shot  # the subtracted Dk, averaged
+    assert abs(ratio / c_mean - 1) < 0.02
+    assert abs(np.sum(w * xi[b, 3]) / np.sum(w)) < 0.02 * shot

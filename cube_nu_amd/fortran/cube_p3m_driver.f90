@@ -3,6 +3,8 @@
 !
 ! Usage: cube_p3m_driver <dir> <ng> <np> <a_mid> <dt> <dt_old> <dt_drift>
 !   or   cube_p3m_driver --version | --sizes
+!   or   cube_p3m_driver --write-header <file>   (a sim_header with known values,
+!        written the way checkpoint.f90:22 writes it)
 !
 ! Reads the buffered layout from raw stream files in <dir> (the format of the
 ! reference's checkpoint streams, checkpoint.f90:11-33): xp.bin (int16,
@@ -32,8 +34,26 @@ program cube_p3m_driver
   type(cube_drift_in) :: din
   type(cube_drift_out) :: dout
   integer(c_int) :: st
+  ! sim_header of parameters.f90:183-210 (not bind(C): written component by component)
+  type sim_header
+    integer(8) nplocal, npglobal, izipx, izipv, image, nn, nnt, nt, ncell, ncb, timestep, cur_checkpoint, &
+        cur_halofind, cur_powerpoint, calculate_PK, cic_iapm
+    real a, t, tau, dt_pp, dt_pm3, dt_pm2, dt_pm1, dt_vmax, mass_p_cdm, m_nu(3), Mass_nu, box, h0, omega_m, &
+        omega_nu, omega_l, s8, vsim2phys, sigma_vres, sigma_vi, z_i, vz_max
+  endtype
+  type(sim_header) :: sim
 
   call get_command_argument(1, arg)
+  if (trim(arg) == '--write-header') then
+    call get_command_argument(2, arg)
+    sim = sim_header(1234567_8, 1234567_8, 2, 2, 1, 1, 4, 8, 32, 4, 17, 3, 0, 0, 0, 0, &
+                     0.25, 1.5, 2.5, 0.01, 0.02, 0.03, 0.04, 0.05, 1.0, [0.02, 0.02, 0.02], 0.06, 600.0, 0.6817, &
+                     0.301, 0.0, 0.699, 0.8, 12.5, 0.0, 3.5, 49.0, 4.3)
+    open(newunit=u, file=trim(arg), access='stream', form='unformatted', status='replace')
+    write(u) sim
+    close(u)
+    stop
+  endif
   if (trim(arg) == '--version') then
     print '(a)', c_string(cube_p3m_version())
     stop

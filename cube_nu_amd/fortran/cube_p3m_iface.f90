@@ -148,6 +148,15 @@ module cube_p3m_iface
       integer(c_int64_t), intent(out) :: np_ext
     end function
 
+    integer(c_int) function cube_p3m_cicpower(ctx, nw, xi, nbin, delta, mean) bind(C, name='cube_p3m_cicpower')
+      import :: c_int, c_ptr, c_int32_t, c_float, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nw, nbin
+      real(c_float), intent(out) :: xi(*)
+      type(c_ptr), value :: delta          ! real(4) delta(nw,nw,nw) via c_loc, or c_null_ptr
+      real(c_double), intent(out) :: mean
+    end function
+
     integer(c_int) function cube_p3m_synchronize(ctx) bind(C, name='cube_p3m_synchronize')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -301,9 +301,19 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
     L.c_range = f_ra_coef<float>(L.pp_range);
     L.inv_app = 1.0f / L.app;
     L.inv_range = 1.0f / L.pp_range;
+    L.two_inv_range = 2.0f / L.pp_range;
+    L.app2 = L.app * L.app;
     L.mass_scale = mass_scale;
     L.v_scale = v_scale;
-    const int64_t m = L.npgrid + 2, m3 = m * m * m;
+    L.inv_rcs = 1.0f / (float)L.rcs;
+    {
+      // r_pp in PP-cell units plus a margin of 1e-3 cell, far above the FP32
+      // rounding of the coordinates, so the windows never cut a pair in range
+      const double rc = (double)L.pp_range * L.sf / L.rcs + 1e-3;
+      L.rwin = (float)(kSX * rc);
+      L.rcut2 = (float)(rc * rc);
+    }
+    const int64_t m = L.npgrid + 2, m3 = m * m * m * kSX;
     const int64_t budget = ws_budget();
     int64_t batch = std::max<int64_t>(1, std::min<int64_t>((int64_t)bx.size(), budget / 4 / (m3 * 12)));
     for (size_t b0 = 0; b0 < bx.size(); b0 += (size_t)batch) {
@@ -320,7 +330,10 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
         KScope k(&tm, "pp_bin_count", 0);
         total = pp_bin_count((int)g.izipx, L, ppboxes.p, nb, xpc(), rhocc(), off.p, S, s);
       }
-      S.entries = ppent.ensure((size_t)std::max<int64_t>(total, 1));
+      // the force kernel queues 31-bit entry indices
+      if (total > (int64_t)INT32_MAX - kPPPad)
+        throw Error(CUBE_ERR_ARG, "PP batch of " + std::to_string(total) + " entries exceeds 2^31; lower workspace_bytes");
+      S.entries = ppent.ensure((size_t)(total + kPPPad));
       S.capacity = total;
       {
         KScope k(&tm, "pp_bin_fill", (double)total * 16);
@@ -328,7 +341,7 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
       }
       {
         KScope k(&tm, "pp_force", (double)total * 16);
-        pp_force(L, prec, nb, vpc(), S, o, s);
+        pp_force(L, prec, ppboxes.p, nb, vpc(), S, o, s);
       }
     }
   }

@@ -4,6 +4,10 @@
 
 namespace cube {
 
+// PP cells are split into kSX sub-bins along x in the binned CSR, so that the
+// partners of a target along a (dy, dz) column are one short contiguous run.
+constexpr int kSX = 4;
+
 struct PPLevel {
   int sf;          // ratio_sf(iapm) = rcp
   int npgrid;      // ntt * rcp
@@ -11,7 +15,12 @@ struct PPLevel {
   float pp_range, pp_range2, app;
   float c_app, c_range;      // (1./35.)/apm**2 for app and pp_range
   float inv_app, inv_range;  // 1/app, 1/pp_range
+  float two_inv_range;       // 2/pp_range
+  float app2;                // app^2
   float mass_scale, v_scale;
+  float inv_rcs;   // 1/rcs: fine -> coarse units (exact for a power-of-two rcs)
+  float rwin;      // kSX * (r_pp in PP-cell units + margin): x window half-width in sub-bins
+  float rcut2;     // (r_pp in PP-cell units + margin)^2: y/z column cut
 };
 
 struct PPBox {
@@ -28,9 +37,9 @@ struct PPOut {
 
 // Scratch the caller provides for one level batch.
 struct PPScratch {
-  int32_t* count = nullptr;   // nbox * m^3
-  int64_t* start = nullptr;   // nbox * m^3 + 1
-  float4* entries = nullptr;  // capacity entries
+  int32_t* count = nullptr;   // nbox * m^3 * kSX sub-bins
+  int64_t* start = nullptr;   // nbox * m^3 * kSX + 1
+  float4* entries = nullptr;  // capacity entries + kPPPad readable padding
   int64_t capacity = 0;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
@@ -42,6 +51,10 @@ int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nb
                      const int32_t* rhoc, const int64_t* cell_off, PPScratch& S, hipStream_t s);
 void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nbox, const void* xp, const int32_t* rhoc,
                  const int64_t* cell_off, PPScratch& S, hipStream_t s);
-void pp_force(const PPLevel& L, int precision, int nbox, float* vp, PPScratch& S, PPOut* out, hipStream_t s);
+void pp_force(const PPLevel& L, int precision, const PPBox* boxes_dev, int nbox, float* vp, PPScratch& S, PPOut* out,
+              hipStream_t s);
+
+// Entries past the last one that the force kernel may read (and ignore).
+constexpr int kPPPad = 8;
 
 }  // namespace cube

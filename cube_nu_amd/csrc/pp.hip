@@ -21,17 +21,21 @@ namespace {
 template <int IZ>
 using XT = typename XP<IZ>::T;
 
-// PP cell of a particle, kick.f90:420-431 in FP32.
+// PP cell of a particle, kick.f90:420-431 in FP32, and its x sub-bin: the
+// PP cell idx covers xv in [idx-1, idx), and sub-bin q = floor(kSX * frac(xv))
+// (exact: xv - floor(xv) and the power-of-two product are exact in FP32).
 template <int IZ>
 __device__ inline bool pp_cell(const PPLevel& L, const PPBox& B, int ci, int cj, int ck, const XT<IZ>* xp, int64_t ip,
-                               float xf[3], int idx[3]) {
+                               float xf[3], int idx[3], int& q) {
   const int ijk[3] = {ci, cj, ck};
   bool in = true;
   for (int d = 0; d < 3; ++d) {
     float xvec = (float)((double)(ijk[d] - 1) + XP<IZ>::off(xp, ip, d));
     xf[d] = xvec * (float)L.rcs;
     float xv = (float)L.sf * (xvec - (float)(B.ifs[d] * L.ntt));
-    idx[d] = (int)floorf(xv) + 1;
+    const float fl = floorf(xv);
+    idx[d] = (int)fl + 1;
+    if (d == 0) q = min((int)((xv - fl) * (float)kSX), kSX - 1);
     in = in && idx[d] >= 0 && idx[d] <= L.npgrid + 1;
   }
   return in;
@@ -40,111 +44,123 @@ __device__ inline bool pp_cell(const PPLevel& L, const PPBox& B, int ci, int cj,
 // One wavefront per (box, coarse cell of the sub-tile + 1-cell ring), one
 // particle per lane.  A PP cell lies inside one coarse cell (idx =
 // floor(sf*(xvec - ifs*ntt)) + 1 with xvec < c), so the wavefront owns the
-// counts of the sf^3 PP cells of its coarse cell and keeps them in LDS: for
-// each distinct PP cell among 64 particles (one ballot per cell) the lanes
-// get their rank from a population count, so the in-cell order is the
-// reference's coarse-cell particle order.  FILL=0 writes the counts, FILL=1
-// writes the entries at start + rank.
+// counts of the nsb = sf^3 * kSX sub-bins of its coarse cell and keeps them in
+// LDS (dynamic, nsb ints per wavefront).  The lanes holding the same sub-bin
+// key are found with one ballot per key bit; a lane's rank among them is a
+// population count, so within a sub-bin the order is the coarse-cell particle
+// order, and the lowest such lane advances the sub-bin's LDS counter (one
+// writer per key).  FILL=0 writes the counts, FILL=1 the entries at
+// start + rank.  Sub-bins are numbered ((z*m + y)*m + x)*kSX + q: each (y, z)
+// row of the PP grid is one contiguous, x-ordered run of the CSR.
 template <int IZ, int FILL>
 __global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restrict__ boxes, int nbox,
                                                 const XT<IZ>* __restrict__ xp, const int32_t* __restrict__ rhoc,
                                                 const int64_t* __restrict__ off, int32_t* __restrict__ count,
                                                 const int64_t* __restrict__ start, float4* __restrict__ ent) {
-  __shared__ int lcnt[4][512];  // sf^3 <= 512 (ratio_sf <= 8)
+  extern __shared__ int lcnt[];
   const int nr = L.ntt + 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (wv >= (int64_t)nbox * nr * nr * nr) return;
   const int ib = (int)(wv / (nr * nr * nr));
-  const int q = (int)(wv % (nr * nr * nr));
+  const int qc = (int)(wv % (nr * nr * nr));
   const PPBox B = boxes[ib];
-  const int c[3] = {B.ifs[0] * L.ntt + (q % nr), B.ifs[1] * L.ntt + (q / nr) % nr,
-                    B.ifs[2] * L.ntt + q / (nr * nr)};  // nc1-1 + offset (Fortran nc1 = ifs*ntt+1)
+  const int c[3] = {B.ifs[0] * L.ntt + (qc % nr), B.ifs[1] * L.ntt + (qc / nr) % nr,
+                    B.ifs[2] * L.ntt + qc / (nr * nr)};  // nc1-1 + offset (Fortran nc1 = ifs*ntt+1)
   const int64_t cell = rc_index(c[0], c[1], c[2], B.T, L.nte, L.ncb);
   const int cnt = rhoc[cell];
   const int64_t base = off[cell];
   const int m = L.npgrid + 2;
-  const int sf = L.sf, sf3 = sf * sf * sf;
+  const int sf = L.sf, nsb = sf * sf * sf * kSX;
+  const int kbits = 32 - __clz(nsb - 1);  // bits of a sub-bin key
   int lo[3];  // PP index of the coarse cell's first PP cell per axis
   for (int d = 0; d < 3; ++d) lo[d] = sf * (c[d] - 1 - B.ifs[d] * L.ntt) + 1;
-  int* lc = lcnt[wave];
-  for (int t = lane; t < sf3; t += 64) lc[t] = 0;
+  int* lc = lcnt + wave * nsb;
+  for (int t = lane; t < nsb; t += 64) lc[t] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int64_t cbase = (int64_t)ib * m * m * m;
+  const unsigned long long lt = (1ull << lane) - 1ull;
   for (int l0 = 0; l0 < cnt; l0 += 64) {
     const int l = l0 + lane;
     float xf[3] = {0, 0, 0};
     int idx[3] = {0, 0, 0};
+    int q = 0;
     bool in = false;
-    if (l < cnt) in = pp_cell<IZ>(L, B, c[0], c[1], c[2], xp, base + l, xf, idx);
-    const int ls = in ? ((idx[2] - lo[2]) * sf + (idx[1] - lo[1])) * sf + (idx[0] - lo[0]) : -1;
-    bool todo = in;
-    while (true) {
-      const unsigned long long act = __ballot(todo);
-      if (act == 0) break;
-      const int leader = __ffsll((long long)act) - 1;
-      const int lls = __shfl(ls, leader, 64);
-      const unsigned long long mk = __ballot(todo && ls == lls);
-      const int prev = lc[lls];  // wave-uniform read
-      if (todo && ls == lls) {
-        if (FILL) {
-          const int64_t pc = cbase + ((int64_t)idx[2] * m + idx[1]) * m + idx[0];
-          const int64_t pos = start[pc] + prev + (int)__popcll(mk & ((1ull << lane) - 1ull));
-          ent[pos] = make_float4(xf[0], xf[1], xf[2], __uint_as_float((unsigned)(base + l)));
-        }
-        todo = false;
-      }
-      __builtin_amdgcn_wave_barrier();
-      if (lane == leader) lc[lls] = prev + (int)__popcll(mk);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (l < cnt) in = pp_cell<IZ>(L, B, c[0], c[1], c[2], xp, base + l, xf, idx, q);
+    const int key = in ? (((idx[2] - lo[2]) * sf + (idx[1] - lo[1])) * sf + (idx[0] - lo[0])) * kSX + q : 0;
+    unsigned long long eq = __ballot(in);
+    for (int bt = 0; bt < kbits; ++bt) {
+      const unsigned long long mb = __ballot((key >> bt) & 1);
+      eq &= ((key >> bt) & 1) ? mb : ~mb;
     }
+    const int prev = in ? lc[key] : 0;
+    if (in) {
+      const int rank = (int)__popcll(eq & lt);
+      if (FILL) {
+        const int64_t pc = cbase + ((int64_t)idx[2] * m + idx[1]) * m + idx[0];
+        ent[start[pc * kSX + q] + prev + rank] =
+            make_float4(xf[0], xf[1], xf[2], __uint_as_float((unsigned)(base + l)));
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (in && (eq & lt) == 0) lc[key] = prev + (int)__popcll(eq);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   if (!FILL) {
-    for (int t = lane; t < sf3; t += 64) {
+    for (int t = lane; t < nsb; t += 64) {
       const int n = lc[t];
       if (n == 0) continue;
-      const int ix = lo[0] + t % sf, iy = lo[1] + (t / sf) % sf, iz = lo[2] + t / (sf * sf);
-      count[cbase + ((int64_t)iz * m + iy) * m + ix] = n;
+      const int pc = t / kSX;
+      const int ix = lo[0] + pc % sf, iy = lo[1] + (pc / sf) % sf, iz = lo[2] + pc / (sf * sf);
+      count[(cbase + ((int64_t)iz * m + iy) * m + ix) * kSX + t % kSX] = n;
     }
   }
 }
 
 constexpr int PB = 4;       // physical PP cells per brick per dim
-constexpr int kUnroll = 8;  // candidate loads in flight per lane
+constexpr int kUnroll = kPPPad;  // candidate loads in flight per lane
 
 
-// FP32 pair law: F(r, app) - F(r, r_pp) with the
-// polynomials in Horner form (FMA) and both branches of F(r, r_pp) evaluated
-// without divergence; r < r_pp for every queued pair, so F(r, r_pp) is one of
-// the two polynomials.  F(r, app) = 1/r^2 except for r <= app (rare).
+// FP32 pair law F(r, app) - F(r, r_pp) for 0 < r < r_pp.  F(r, r_pp) is one
+// of the two polynomials of F_ra (kick.f90:360-374), both evaluated in Horner
+// form so that lanes do not diverge.  For r > app (all but vanishingly few
+// pairs) F(r, app) = 1/r^2, and the outer branch's c*12/ep^2 term, which
+// equals (3/35)/r^2 for c = (1/35)/r_pp^2 and ep = 2r/r_pp, is folded into
+// it: 1/r^2 - c (12/ep^2 + P(ep)) = (32/35)/r^2 - c P(ep).  The rare r <= app
+// pair takes the full forms.  The error of a pair term stays at the level of
+// FP32 rounding.
 __device__ inline float pair_law_fast(const PPLevel& L, float r2, float& rinv_out) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * rinv;
-  float fa = rinv * rinv;
-  const float ea = (2.0f * r) * L.inv_app;
-  if (ea <= 2.0f) {
-    const float e2 = ea * ea;
-    fa = ea > 1.0f
-             ? L.c_app * (12.0f * __builtin_amdgcn_rcpf(e2) +
-                          fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, 7.0f, -48.0f), 70.0f), 224.0f),
-                                                 -840.0f),
-                                        896.0f),
-                               -224.0f))
-             : L.c_app * ea * fmaf(e2, fmaf(ea, fmaf(ea, fmaf(ea, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
-  }
-  const float ep = (2.0f * r) * L.inv_range;
+  const float rinv2 = rinv * rinv;
+  const float ep = r * L.two_inv_range;
   const float e2 = ep * ep;
-  const float outer =
-      12.0f * __builtin_amdgcn_rcpf(e2) +
+  const float po =
       fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, -48.0f), 70.0f), 224.0f), -840.0f), 896.0f), -224.0f);
-  const float inner = ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
-  const float fr = L.c_range * (ep > 1.0f ? outer : inner);
+  const float pi = ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+  const bool outer = ep > 1.0f;
+  float s = fmaf(outer ? 32.0f / 35.0f : 1.0f, rinv2, -L.c_range * (outer ? po : pi));
+  if (r2 <= L.app2) {  // ep(r, app) <= 2
+    const float ea = (2.0f * r) * L.inv_app;
+    const float a2 = ea * ea;
+    const float fa =
+        ea > 1.0f
+            ? L.c_app * (12.0f * __builtin_amdgcn_rcpf(a2) +
+                         fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, fmaf(ea, 7.0f, -48.0f), 70.0f), 224.0f),
+                                                -840.0f),
+                                       896.0f),
+                              -224.0f))
+            : L.c_app * ea * fmaf(a2, fmaf(ea, fmaf(ea, fmaf(ea, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
+    s = fa - L.c_range * (outer ? 12.0f * __builtin_amdgcn_rcpf(e2) + po : pi);
+  }
   rinv_out = rinv;
-  return fa - fr;
+  return s;
 }
 
 // Pair force of candidate o on target me (kick.f90:523-530, 554-560):
@@ -164,9 +180,9 @@ __device__ inline void pair_force<float>(const PPLevel& L, const float4 me, cons
   float rinv;
   const float s = pair_law_fast(L, r2, rinv);
   const float sc = rinv * s;
-  f1[0] = f1[0] + rv0 * sc;
-  f1[1] = f1[1] + rv1 * sc;
-  f1[2] = f1[2] + rv2 * sc;
+  f1[0] = fmaf(rv0, sc, f1[0]);
+  f1[1] = fmaf(rv1, sc, f1[1]);
+  f1[2] = fmaf(rv2, sc, f1[2]);
 }
 
 template <>
@@ -183,21 +199,24 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4 me, con
 
 // One workgroup per brick of PB^3 physical PP cells; each thread owns target
 // particles and sums the pair forces of the 27 neighbouring PP cells in the
-// fixed (dz, dy, dx, p) order.  The (PB+2)^3 neighbourhood's counts and
-// starts are staged in LDS once; the three cells of each (dy, dz) column are
-// one contiguous run of candidates, read kUnroll at a time so that many loads
-// are in flight per lane (they come from L1/L2: neighbouring targets read the
-// same candidates).  Candidates stay in the caches: staging them in LDS
-// measured slower (64 ms vs 48 ms at 512^3).
+// fixed (dz, dy, x) order.  PP cells are split into kSX sub-bins along x and
+// the CSR is x-ordered along each (y, z) row, so the part of a (dy, dz)
+// column that can hold partners, x in [x_t - r_pp, x_t + r_pp], is one
+// contiguous run found from two sub-bin starts; columns whose y/z distance
+// from the target exceeds r_pp are skipped.  Both windows carry a margin far
+// above FP32 rounding, so every pair the reference tests in range is tested
+// here with the reference's FP32 operations and the interacting set is the
+// reference's.  The sub-bin starts of the (PB+2)^3 neighbourhood are staged
+// in LDS; candidates are read kUnroll at a time from L1/L2 (the entry array
+// is padded by kUnroll so the reads need no clamp).
 template <typename R>
-__global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __restrict__ count,
+__global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
                                                   PPOut* __restrict__ out) {
-  constexpr int NB = PB + 2, NB3 = NB * NB * NB;
+  constexpr int NB = PB + 2, RW = kSX * NB + 1;  // staged sub-bin starts per (y, z) row
   __shared__ int pre[PB * PB * PB + 1];
-  __shared__ int ncnt[NB3];
-  __shared__ int64_t nstart[NB3];
+  __shared__ int64_t rs[NB * NB * RW];
   __shared__ float red[4];
   __shared__ unsigned long long cnt_red[2];
   const int nbr = (L.npgrid + PB - 1) / PB;
@@ -209,20 +228,19 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __re
   const int bz = (int)(b % nbr);
   const int ib = (int)(b / nbr);
   const int m = L.npgrid + 2;
+  const int64_t mS = (int64_t)m * kSX;
   const int64_t cbase = (int64_t)ib * m * m * m;
   const int c0[3] = {1 + bx * PB, 1 + by * PB, 1 + bz * PB};
-  // neighbourhood cells c0-1 .. c0+PB (clipped to the PP grid 0..npgrid+1)
-  for (int q = threadIdx.x; q < NB3; q += blockDim.x) {
-    const int x = c0[0] - 1 + q % NB, y = c0[1] - 1 + (q / NB) % NB, z = c0[2] - 1 + q / (NB * NB);
-    int c = 0;
+  // rows (y, z) = c0-1 .. c0+PB, sub-bins kSX*(c0x-1) .. kSX*(c0x+PB+1); rows
+  // outside the PP grid are empty, sub-bins past the row end read the row end
+  for (int q = threadIdx.x; q < NB * NB * RW; q += blockDim.x) {
+    const int xs = q % RW, y = c0[1] - 1 + (q / RW) % NB, z = c0[2] - 1 + q / (RW * NB);
     int64_t st = 0;
-    if (x <= L.npgrid + 1 && y <= L.npgrid + 1 && z <= L.npgrid + 1) {
-      const int64_t nc = cbase + ((int64_t)z * m + y) * m + x;
-      c = count[nc];
-      st = start[nc];
+    if (y <= L.npgrid + 1 && z <= L.npgrid + 1) {
+      const int64_t row = (cbase + ((int64_t)z * m + y) * m) * kSX;
+      st = start[row + min((int64_t)kSX * (c0[0] - 1) + xs, mS)];
     }
-    ncnt[q] = c;
-    nstart[q] = st;
+    rs[q] = st;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -230,14 +248,18 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __re
     for (int q = 0; q < PB * PB * PB; ++q) {
       pre[q] = s;
       const int lx = q % PB, ly = (q / PB) % PB, lz = q / (PB * PB);
-      if (c0[0] + lx <= L.npgrid && c0[1] + ly <= L.npgrid && c0[2] + lz <= L.npgrid)
-        s += ncnt[((lz + 1) * NB + ly + 1) * NB + lx + 1];
+      if (c0[0] + lx <= L.npgrid && c0[1] + ly <= L.npgrid && c0[2] + lz <= L.npgrid) {
+        const int row = ((lz + 1) * NB + ly + 1) * RW;
+        s += (int)(rs[row + kSX * (lx + 2)] - rs[row + kSX * (lx + 1)]);
+      }
     }
     pre[PB * PB * PB] = s;
     cnt_red[0] = 0;
     cnt_red[1] = 0;
   }
   __syncthreads();
+  const PPBox B = boxes[ib];
+  const float xoff[3] = {(float)(B.ifs[0] * L.ntt), (float)(B.ifs[1] * L.ntt), (float)(B.ifs[2] * L.ntt)};
   const int ntg = pre[PB * PB * PB];
   float f2 = 0, vm[3] = {0, 0, 0};
   unsigned long long npp = 0, npg = 0;
@@ -249,37 +271,59 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __re
       else hi = mid - 1;
     }
     const int lx = lo % PB, ly = (lo / PB) % PB, lz = lo / (PB * PB);
-    const int tq = ((lz + 1) * NB + ly + 1) * NB + lx + 1;
-    const float4 me = ent[nstart[tq] + (t - pre[lo])];
+    const int rowt = ((lz + 1) * NB + ly + 1) * RW;
+    const float4 me = ent[rs[rowt + kSX * (lx + 1)] + (t - pre[lo])];
+    // PP coordinates of the target, the binning's xv = sf (xvec - ifs ntt)
+    // (xvec = xf / rcs exactly for a power-of-two rcs)
+    const float xme[3] = {me.x, me.y, me.z};
+    float fr[3];
+    for (int d = 0; d < 3; ++d) {
+      const float xv = (float)L.sf * (xme[d] * L.inv_rcs - xoff[d]);
+      fr[d] = xv - (float)(c0[d] - 1 + (d == 0 ? lx : (d == 1 ? ly : lz)));  // offset in its PP cell
+    }
+    // x window in local sub-bins: cells lx .. lx+2 of the staged rows
+    const float u = (fr[0] + (float)(lx + 1)) * (float)kSX;
+    const int wlo = max((int)floorf(u - L.rwin), kSX * lx);
+    const int whi = min((int)floorf(u + L.rwin), kSX * (lx + 3) - 1);
     R f1[3] = {0, 0, 0};
-    // PP cells are numbered x fastest, so the cells x-1, x, x+1 of a (dy, dz)
-    // column are one contiguous run of the CSR: 9 runs per target
+    // pairs in range of this target, and those with a ghost partner (32-bit)
+    int npair = 0, nghost = 0;
     const bool gx0 = c0[0] + lx - 1 == 0, gx1 = c0[0] + lx + 1 == L.npgrid + 1;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy) {
+        const float ey = dy < 0 ? fr[1] : (dy > 0 ? 1.0f - fr[1] : 0.0f);
+        const float ez = dz < 0 ? fr[2] : (dz > 0 ? 1.0f - fr[2] : 0.0f);
+        if (ey * ey + ez * ez >= L.rcut2) continue;
         const int ny = c0[1] + ly + dy, nz = c0[2] + lz + dz;
         const bool gyz = ny == 0 || nz == 0 || ny == L.npgrid + 1 || nz == L.npgrid + 1;
-        const int q0 = tq + (dz * NB + dy) * NB - 1;  // cell x-1 of the column
-        const int n0 = ncnt[q0], n1 = ncnt[q0 + 1];
-        const int nn = n0 + n1 + ncnt[q0 + 2];
-        const float4* run = ent + nstart[q0];
+        const int row = rowt + (dz * NB + dy) * RW;
+        const int64_t beg = rs[row + wlo];
+        const int nn = (int)(rs[row + whi + 1] - beg);
+        // candidates [ph0, ph1) of the window are physical: the ghost cells
+        // x-1 / x+1 lie before / after them, a ghost row holds none
+        int ph0 = gx0 ? max((int)(rs[row + kSX * (lx + 1)] - beg), 0) : 0;
+        int ph1 = gx1 ? min((int)(rs[row + kSX * (lx + 2)] - beg), nn) : nn;
+        if (gyz) ph0 = ph1 = 0;
+        const unsigned phn = (unsigned)(ph1 - ph0);
+        const float4* run = ent + beg;
         for (int p = 0; p < nn; p += kUnroll) {
           float4 o[kUnroll];
 #pragma unroll
-          for (int k = 0; k < kUnroll; ++k) o[k] = run[min(p + k, nn - 1)];
+          for (int k = 0; k < kUnroll; ++k) o[k] = run[p + k];
 #pragma unroll
           for (int k = 0; k < kUnroll; ++k) {
             const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
             const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
             if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
               pair_force<R>(L, me, o[k], f1);
-              const int i = p + k;
-              if (gyz || (gx0 && i < n0) || (gx1 && i >= n0 + n1)) ++npg;
-              else ++npp;
+              ++npair;
+              nghost += (unsigned)(p + k - ph0) >= phn;
             }
           }
         }
       }
+    npg += (unsigned)nghost;
+    npp += (unsigned)(npair - nghost);
     // update v (kick.f90:576-582) in FP32 as the reference does
     const unsigned ip = __float_as_uint(me.w);
     float g[3];
@@ -326,15 +370,16 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const int32_t* __re
 int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
                      const int64_t* off, PPScratch& S, hipStream_t s) {
   const int m = L.npgrid + 2;
-  const int64_t ncell = (int64_t)nbox * m * m * m;
+  const int64_t ncell = (int64_t)nbox * m * m * m * kSX;
   zero_async(S.count, sizeof(int32_t) * ncell, s);
   const int nr = L.ntt + 2;
   const int64_t nthr = (int64_t)nbox * nr * nr * nr * 64;
+  const size_t lds = sizeof(int) * 4 * L.sf * L.sf * L.sf * kSX;
   if (izipx == 2)
-    hipLaunchKernelGGL((k_pp_bin<2, 0>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
+    hipLaunchKernelGGL((k_pp_bin<2, 0>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), lds, s, L, boxes, nbox,
                        (const int16_t*)xp, rhoc, off, S.count, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_pp_bin<1, 0>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
+    hipLaunchKernelGGL((k_pp_bin<1, 0>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), lds, s, L, boxes, nbox,
                        (const int8_t*)xp, rhoc, off, S.count, nullptr, nullptr);
   CUBE_HIP(hipGetLastError());
   scan_counts(S.count, S.start, ncell, S.scan_tmp, S.scan_tmp_bytes, s);
@@ -348,23 +393,25 @@ void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, cons
                  const int64_t* off, PPScratch& S, hipStream_t s) {
   const int nr = L.ntt + 2;
   const int64_t nthr = (int64_t)nbox * nr * nr * nr * 64;
+  const size_t lds = sizeof(int) * 4 * L.sf * L.sf * L.sf * kSX;
   if (izipx == 2)
-    hipLaunchKernelGGL((k_pp_bin<2, 1>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
+    hipLaunchKernelGGL((k_pp_bin<2, 1>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), lds, s, L, boxes, nbox,
                        (const int16_t*)xp, rhoc, off, S.count, S.start, S.entries);
   else
-    hipLaunchKernelGGL((k_pp_bin<1, 1>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
+    hipLaunchKernelGGL((k_pp_bin<1, 1>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), lds, s, L, boxes, nbox,
                        (const int8_t*)xp, rhoc, off, S.count, S.start, S.entries);
   CUBE_HIP(hipGetLastError());
 }
 
-void pp_force(const PPLevel& L, int precision, int nbox, float* vp, PPScratch& S, PPOut* out, hipStream_t s) {
+void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, float* vp, PPScratch& S, PPOut* out,
+              hipStream_t s) {
   const int nbr = (L.npgrid + PB - 1) / PB;
   const int64_t grid = (int64_t)nbr * nbr * nbr * nbox;
   if (precision == 64)
-    hipLaunchKernelGGL(k_pp_force<double>, dim3((unsigned)grid), dim3(256), 0, s, L, S.count, S.start, S.entries, vp,
+    hipLaunchKernelGGL(k_pp_force<double>, dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.entries, vp,
                        out);
   else
-    hipLaunchKernelGGL(k_pp_force<float>, dim3((unsigned)grid), dim3(256), 0, s, L, S.count, S.start, S.entries, vp,
+    hipLaunchKernelGGL(k_pp_force<float>, dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.entries, vp,
                        out);
   CUBE_HIP(hipGetLastError());
 }

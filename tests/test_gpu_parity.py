@@ -71,6 +71,19 @@ def test_phase_parity(gpu, data, tfs, mask):
         assert out_g["napm"] == out_o["napm"]
 
 
+def test_pp_pair_set_is_the_references(gpu, data, tfs):
+    """The PP kernel tests only the x window and the y/z columns that can hold
+    partners; the number of interacting pairs (physical-physical counted once,
+    physical-ghost once, ghost-ghost never, kick.f90:516-566) must equal the
+    oracle's full 27-cell count exactly."""
+    xp, vp, pid, rhoc = data
+    gpu.kick(xp, vp.copy(), rhoc, a_mid=A_MID, dt=DT, phase_mask=8)
+    n_gpu = gpu.timing_report()["pp_pairs_last"]["calls"]
+    _, _, n_orc = orc.kick(oparams(_geo()), xp, vp.copy(), rhoc, tfs, A_MID, DT, phase_mask=8)
+    print(f"pp pairs gpu {n_gpu} oracle {n_orc}")
+    assert n_orc > 0 and n_gpu == n_orc
+
+
 def test_pm1_density_and_testrho(gpu, data, tfs):
     xp, vp, pid, rhoc = data
     v = vp.copy()

@@ -565,6 +565,182 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
   }
 }
 
+// Interpolation for the tile and sub-tile levels with w = 4 or 8 mesh cells per
+// coarse cell and a power-of-two ratio_cs/gridsize: one workgroup per group
+// of coarse cells (2x2x4 or 1x1x2) whose particles' floor(xpos) fill an
+// IBX x IBY x IBZ block of mesh cells.  For such levels floor(xpos) of a
+// particle of coarse cell i is w(i-1) + floor(w frac) - shift exactly (FP64
+// products with a power of two), so the block's particles are exactly those
+// of its cells: each is read once, unlike a mesh-aligned block, which reads
+// every coarse cell its cells touch.  The staging, FD force, f2max and the
+// 27-point gather are those of k_interp.
+constexpr int IBX = 8, IBY = 8, IBZ = 16;
+
+template <typename R, int IZ>
+__global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __restrict__ boxes,
+                                                      const XT<IZ>* __restrict__ xp, float* __restrict__ vp,
+                                                      const int32_t* __restrict__ rhoc,
+                                                      const int64_t* __restrict__ off, const R* __restrict__ phi,
+                                                      float a_mid, float dt, PMOut* __restrict__ out) {
+  constexpr int FX = IBX + 2, FY = IBY + 2, FZ = IBZ + 2, FN = FX * FY * FZ;
+  constexpr int PX = IBX + 6, PY = IBY + 6, PZ = IBZ + 6, PN = PX * PY * PZ;
+  constexpr int NF = (FN + 255) / 256, NPH = (PN + 255) / 256;
+  constexpr int LDSN = PN > 3 * FN ? PN : 3 * FN;
+  constexpr int MAXC = 16;
+  __shared__ R lds[LDSN];
+  __shared__ R red[4];
+  __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4];
+  __shared__ int64_t gof[MAXC];
+  const int w = L.w;
+  const int g[3] = {IBX / w, IBY / w, IBZ / w};  // coarse cells per block per axis
+  const int ncell = L.nforce / w;                // physical coarse cells of a box per axis
+  const int nb[3] = {(ncell + g[0] - 1) / g[0], (ncell + g[1] - 1) / g[1], (ncell + g[2] - 1) / g[2]};
+  int64_t b = blockIdx.x;
+  const int bx = (int)(b % nb[0]);
+  b /= nb[0];
+  const int by = (int)(b % nb[1]);
+  b /= nb[1];
+  const int bz = (int)(b % nb[2]);
+  const int ibox = (int)(b / nb[2]);
+  const PMBox B = boxes[ibox];
+  const int bi[3] = {bx, by, bz};
+  int clo[3], chi[3], f0[3], fend[3];
+  for (int d = 0; d < 3; ++d) {
+    clo[d] = B.nc1[d] + bi[d] * g[d];
+    chi[d] = min(clo[d] + g[d] - 1, B.nc2[d]);
+    f0[d] = w * (clo[d] - 1) - B.ushift[d];
+    fend[d] = f0[d] + w * (chi[d] - clo[d] + 1);  // last force cell this block counts for f2max
+  }
+  // 1. phi(f0-2 .. f0+IB+3) and the cell metadata, all loads before the
+  //    first LDS store
+  R ph[NPH];
+#pragma unroll
+  for (int q = 0; q < NPH; ++q) {
+    const int t = threadIdx.x + q * 256;
+    ph[q] = (R)0;
+    if (t < PN) {
+      const int l[3] = {t % PX, (t / PX) % PY, t / (PX * PY)};
+      int a[3];
+      bool ok = true;
+      for (int d = 0; d < 3; ++d) {
+        a[d] = f0[d] - 2 + l[d] + B.org[d] - L.m0;
+        ok = ok && a[d] >= 0 && a[d] < L.n;
+      }
+      if (ok) ph[q] = phi[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]];
+    }
+  }
+  const int ncx = chi[0] - clo[0] + 1, ncy = chi[1] - clo[1] + 1, ncz = chi[2] - clo[2] + 1;
+  const int nC = ncx * ncy * ncz;
+  for (int t = threadIdx.x; t < nC; t += blockDim.x) {
+    const int64_t cell = rc_index(clo[0] + t % ncx, clo[1] + (t / ncx) % ncy, clo[2] + t / (ncx * ncy), B.T, L.nte,
+                                  L.ncb);
+    cnt[t] = rhoc[cell];
+    gof[t] = off[cell];
+  }
+#pragma unroll
+  for (int q = 0; q < NPH; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < PN) lds[t] = ph[q];
+  }
+  __syncthreads();
+  block_scan(cnt, pre, nC, wsum);
+  const int total = pre[nC];
+  // 2. force(d, f0 .. f0+IB+1) = sum(phi(m + [-2,-1,1,2]) * weight)
+  const FDWeights<R> W;
+  R fr[NF][3];
+  R f2 = 0;
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < FN) {
+      const int lx = t % FX, ly = (t / FX) % FY, lz = t / (FX * FY);
+      const int px = lx + 2, py = ly + 2, pz = lz + 2;
+      auto P = [&](int x, int y, int z) { return lds[(z * PY + y) * PX + x]; };
+      R sx = 0, sy = 0, sz = 0;
+      sx = sx + P(px - 2, py, pz) * W.w0;
+      sx = sx + P(px - 1, py, pz) * W.w1;
+      sx = sx + P(px + 1, py, pz) * W.w2;
+      sx = sx + P(px + 2, py, pz) * W.w3;
+      sy = sy + P(px, py - 2, pz) * W.w0;
+      sy = sy + P(px, py - 1, pz) * W.w1;
+      sy = sy + P(px, py + 1, pz) * W.w2;
+      sy = sy + P(px, py + 2, pz) * W.w3;
+      sz = sz + P(px, py, pz - 2) * W.w0;
+      sz = sz + P(px, py, pz - 1) * W.w1;
+      sz = sz + P(px, py, pz + 1) * W.w2;
+      sz = sz + P(px, py, pz + 2) * W.w3;
+      fr[q][0] = sx;
+      fr[q][1] = sy;
+      fr[q][2] = sz;
+      // f2max over force cells 1..nforce (kick.f90:717); neighbouring blocks
+      // share their boundary cell, which is harmless for a maximum
+      const int m[3] = {f0[0] + lx, f0[1] + ly, f0[2] + lz};
+      bool in = true;
+      for (int d = 0; d < 3; ++d) in = in && (m[d] >= 1 && m[d] <= L.nforce && m[d] <= fend[d]);
+      if (in) f2 = max(f2, (sx * sx + sy * sy) + sz * sz);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < FN) {
+      lds[t] = fr[q][0];
+      lds[FN + t] = fr[q][1];
+      lds[2 * FN + t] = fr[q][2];
+    }
+  }
+  __syncthreads();
+  // 3. every particle of the block's cells
+  constexpr double kRes = 1.0 / (double)(1u << XP<IZ>::B);
+  R vm[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < total; i += 256) {
+    const int c = find_cell(pre, nC, i);
+    const int64_t ip = gof[c] + (i - pre[c]);
+    uint32_t uu[3];
+    float v0[3];
+    for (int d = 0; d < 3; ++d) {
+      uu[d] = XP<IZ>::raw(xp, ip, d);
+      v0[d] = vp[3 * ip + d];
+    }
+    const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+    R wt[3][3];
+    int f[3];
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+      const double xpos = ((double)(ijk[d] - 1) + ((double)uu[d] + 0.5) * kRes) * L.xmul - (double)B.ushift[d];
+      f[d] = tsc_weights<R>(xpos, wt[d]) - f0[d];
+      ok = ok && f[d] >= 0 && f[d] < (d == 0 ? IBX : (d == 1 ? IBY : IBZ));
+    }
+    if (!ok) continue;  // unreachable for a power-of-two xmul; guards the LDS reads
+    R dv[3] = {0, 0, 0};
+    for (int i3 = 0; i3 < 3; ++i3)
+      for (int i2 = 0; i2 < 3; ++i2)
+        for (int i1 = 0; i1 < 3; ++i1) {
+          const int li = ((f[2] + i3) * FY + (f[1] + i2)) * FX + (f[0] + i1);
+          for (int d = 0; d < 3; ++d)
+            dv[d] = dv[d] + ((lds[d * FN + li] * wt[0][i1]) * wt[1][i2]) * wt[2][i3];
+        }
+    for (int d = 0; d < 3; ++d) {
+      const float v = (float)kick_update((R)v0[d], dv[d], a_mid, dt);
+      vp[3 * ip + d] = v;
+      vm[d] = max(vm[d], (R)fabsf(v));
+    }
+  }
+  R bm = block_max(f2, red);
+  if (threadIdx.x == 0 && bm > 0) {
+    if constexpr (std::is_same<R, float>::value) atomic_max_nonneg(&out->f2max, bm);
+    else atomic_max_nonneg(&out->f2max_d, bm);
+  }
+  for (int d = 0; d < 3; ++d) {
+    R m = block_max(vm[d], red);
+    if (threadIdx.x == 0 && m > 0) {
+      if constexpr (std::is_same<R, float>::value) atomic_max_nonneg(&out->vmax[d], m);
+      else atomic_max_nonneg(&out->vmax_d[d], m);
+    }
+  }
+}
+
 // Green_3D, Green/Green.f90:22-71 at nn=1.
 __global__ void k_green(int n, float apm_soft, float apm_range, float gridsize, double factor, float* out) {
   const int nh = n / 2 + 1;
@@ -750,6 +926,16 @@ void launch_interp_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void
 template <typename R, int IZ>
 void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
                    const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
+  if (!L.periodic && !L.xdiv && (L.w == 4 || L.w == 8) && L.nforce % L.w == 0) {
+    const int ncell = L.nforce / L.w;
+    int64_t grid = nbox;
+    const int g[3] = {IBX / L.w, IBY / L.w, IBZ / L.w};
+    for (int d = 0; d < 3; ++d) grid *= (ncell + g[d] - 1) / g[d];
+    hipLaunchKernelGGL((k_interp_cells<R, IZ>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, (const XT<IZ>*)xp,
+                       vp, rhoc, off, phi, a_mid, dt, out);
+    CUBE_HIP(hipGetLastError());
+    return;
+  }
   launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
 }
 

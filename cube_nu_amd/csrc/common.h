@@ -106,6 +106,16 @@ __device__ inline int tsc_weights(double xpos, R w[3]) {
   return (int)fl;
 }
 
+// The same weights from l3 = xpos - floor(xpos) given directly (exact when
+// xpos is a multiple of a power of two, as for power-of-two ratio_cs/gridsize).
+template <typename R>
+__device__ inline void tsc_weights_frac(R l3, R w[3]) {
+  R a = (R)1 - l3;
+  w[0] = (a * a) / (R)2;
+  w[2] = (l3 * l3) / (R)2;
+  w[1] = ((R)1 - w[0]) - w[2];
+}
+
 // FD weights [(alpha-1)/4, -alpha/2, alpha/2, (1-alpha)/4], alpha=4./3
 // (parameters.f90:53-54), evaluated in R like the reference's real(4).
 template <typename R>

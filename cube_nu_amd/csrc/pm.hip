@@ -111,12 +111,10 @@ __device__ inline double pm_xpos(const PMLevel& L, int c, double frac, int shift
   return (L.xdiv ? (a * (double)L.rcs) / L.gridsize : a * L.xmul) - (double)shift;
 }
 
-// Deposits one particle's 27 TSC weights into the block's LDS accumulator.
+// Deposits one particle's 27 TSC weights (floor f, weights w) into the
+// block's LDS accumulator.
 template <typename R, typename Acc, int BD>
-__device__ inline void deposit_particle(Acc* acc, const double xpos[3], const int mlo[3], const int mhi[3]) {
-  R w[3][3];
-  int f[3];
-  for (int d = 0; d < 3; ++d) f[d] = tsc_weights<R>(xpos[d], w[d]);
+__device__ inline void deposit_particle(Acc* acc, const int f[3], const R w[3][3], const int mlo[3], const int mhi[3]) {
   const bool inner = f[0] >= mlo[0] && f[0] + 2 <= mhi[0] && f[1] >= mlo[1] && f[1] + 2 <= mhi[1] &&
                      f[2] >= mlo[2] && f[2] + 2 <= mhi[2];
   for (int i3 = 0; i3 < 3; ++i3) {
@@ -153,7 +151,7 @@ __device__ inline void wave_sync_lds() {
 // power of two), appends hits with their coordinates to a per-wavefront LDS
 // queue, and deposits 64 queued particles at a time with every lane busy and
 // no memory access but LDS atomics.
-template <typename R, int IZ, int BD>
+template <typename R, int IZ, int BD, bool CA>
 __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restrict__ boxes,
                                                  const XT<IZ>* __restrict__ xp, const int32_t* __restrict__ rhoc,
                                                  const int64_t* __restrict__ off, R* __restrict__ mesh, float mass) {
@@ -164,7 +162,12 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   __shared__ int cnt[kMaxFine], pre[kMaxFine + 1], wsum[4];
   __shared__ int64_t gof[kMaxFine];
   __shared__ unsigned long long queue[4][128];
-  const int nblk = (L.win_hi - L.win_lo + 1 + BD - 1) / BD;
+  // CA: blocks aligned to groups of BD/w coarse cells, counted down from the
+  // last deposited cell nc2 + nloop (their regions then end past win_hi and
+  // begin before win_lo); the block reads its own cells and the lower
+  // neighbours only.  Otherwise blocks tile the window from win_lo.
+  const int gcell = BD / L.w;
+  const int nblk = CA ? 1 + (L.nforce / L.w - 1 + 2 * L.nloop + gcell - 1) / gcell : (L.win_hi - L.win_lo + BD) / BD;
   int64_t b = blockIdx.x;
   const int bx = (int)(b % nblk);
   b /= nblk;
@@ -176,8 +179,10 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   const int bidx[3] = {bx, by, bz};
   int mlo[3], mhi[3], clo[3], chi[3];
   for (int d = 0; d < 3; ++d) {
-    mlo[d] = L.win_lo + bidx[d] * BD;
-    mhi[d] = min(mlo[d] + BD - 1, L.win_hi);
+    int m0 = L.win_lo + bidx[d] * BD;
+    if (CA) m0 = L.w * (B.nc2[d] + L.nloop - (nblk - 1 - bidx[d]) * gcell - 1) - B.ushift[d];
+    mlo[d] = max(m0, L.win_lo);
+    mhi[d] = min(m0 + BD - 1, L.win_hi);
     // particle of coarse cell i reaches mesh cells [w(i-1)-s, wi-s+1]
     clo[d] = max(-floor_div(-(mlo[d] - 1 + B.ushift[d]), L.w), B.nc1[d] - L.nloop);
     chi[d] = min(floor_div(mhi[d] + B.ushift[d], L.w) + 1, B.nc2[d] + L.nloop);
@@ -189,6 +194,7 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   unsigned long long* q = queue[wave];
   // the floor index as integer arithmetic: shift = B - log2(w)
   const int sh = L.xdiv ? -1 : XP<IZ>::B - (31 - __clz(L.w));
+  const R rsh = sh >= 0 ? (R)1 / (R)(1u << sh) : (R)0;
   constexpr double kRes = 1.0 / (double)(1u << XP<IZ>::B);
   auto process = [&](int nq) {
     wave_sync_lds();
@@ -196,12 +202,20 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
       const unsigned long long e = q[lane];
       const int c = (int)(e >> 48);
       const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
-      double xpos[3];
+      int f[3];
+      R wt[3][3];
       for (int d = 0; d < 3; ++d) {
         const uint32_t u = (uint32_t)(e >> (16 * d)) & 0xFFFFu;
-        xpos[d] = pm_xpos(L, ijk[d], ((double)u + 0.5) * kRes, B.ushift[d]);
+        if (sh >= 0) {
+          // xpos = w(c-1) - s + (u + 1/2) 2^-sh exactly: floor and fraction
+          // from the integer coordinate, the fraction exact in R
+          f[d] = L.w * (ijk[d] - 1) - B.ushift[d] + (int)(u >> sh);
+          tsc_weights_frac<R>(((R)(u & ((1u << sh) - 1u)) + (R)0.5) * rsh, wt[d]);
+        } else {
+          f[d] = tsc_weights<R>(pm_xpos(L, ijk[d], ((double)u + 0.5) * kRes, B.ushift[d]), wt[d]);
+        }
       }
-      deposit_particle<R, Acc, BD>(acc, xpos, mlo, mhi);
+      deposit_particle<R, Acc, BD>(acc, f, wt, mlo, mhi);
     }
   };
   int qn = 0;  // wave-uniform queue length
@@ -568,10 +582,10 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
 // Interpolation for the tile and sub-tile levels with w = 4 or 8 mesh cells per
 // coarse cell and a power-of-two ratio_cs/gridsize: one workgroup per group
 // of coarse cells (2x2x4 or 1x1x2) whose particles' floor(xpos) fill an
-// IBX x IBY x IBZ block of mesh cells.  For such levels floor(xpos) of a
-// particle of coarse cell i is w(i-1) + floor(w frac) - shift exactly (FP64
-// products with a power of two), so the block's particles are exactly those
-// of its cells: each is read once, unlike a mesh-aligned block, which reads
+// IBX x IBY x IBZ block of mesh cells.  For such levels xpos of a particle
+// of coarse cell i is w(i-1) - shift + (u + 1/2) 2^-sh exactly (sh = B -
+// log2 w), so floor and fraction come from the integer coordinate and the
+// block's particles are exactly those of its cells: each is read once, unlike a mesh-aligned block, which reads
 // every coarse cell its cells touch.  The staging, FD force, f2max and the
 // 27-point gather are those of k_interp.
 constexpr int IBX = 8, IBY = 8, IBZ = 16;
@@ -692,7 +706,8 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
   }
   __syncthreads();
   // 3. every particle of the block's cells
-  constexpr double kRes = 1.0 / (double)(1u << XP<IZ>::B);
+  const int sh = XP<IZ>::B - (31 - __clz(w));
+  const R rsh = (R)1 / (R)(1u << sh);
   R vm[3] = {0, 0, 0};
   for (int i = threadIdx.x; i < total; i += 256) {
     const int c = find_cell(pre, nC, i);
@@ -708,8 +723,9 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
     int f[3];
     bool ok = true;
     for (int d = 0; d < 3; ++d) {
-      const double xpos = ((double)(ijk[d] - 1) + ((double)uu[d] + 0.5) * kRes) * L.xmul - (double)B.ushift[d];
-      f[d] = tsc_weights<R>(xpos, wt[d]) - f0[d];
+      // xpos = w(c-1) - s + (u + 1/2) 2^-sh exactly (sh = B - log2 w)
+      f[d] = L.w * (ijk[d] - 1) - B.ushift[d] + (int)(uu[d] >> sh) - f0[d];
+      tsc_weights_frac<R>(((R)(uu[d] & ((1u << sh) - 1u)) + (R)0.5) * rsh, wt[d]);
       ok = ok && f[d] >= 0 && f[d] < (d == 0 ? IBX : (d == 1 ? IBY : IBZ));
     }
     if (!ok) continue;  // unreachable for a power-of-two xmul; guards the LDS reads
@@ -890,10 +906,21 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
 template <typename R, int IZ, int BD>
 void launch_deposit_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
                        const int64_t* off, R* mesh, float mass, hipStream_t s) {
-  const int nblk = (L.win_hi - L.win_lo + 1 + BD - 1) / BD;
+  // Cell-aligned blocks when w divides BD, w < BD/2 and the window is the
+  // deposited cells' floors plus one cell (win_lo = 1 - nloop w,
+  // win_hi = nforce + nloop w): the regions, counted down from the last
+  // deposited cell, then cover it.
+  const bool ca = !L.xdiv && 2 * L.w < BD && BD % L.w == 0 && L.nforce % L.w == 0 &&
+                  L.win_lo == 1 - L.nloop * L.w && L.win_hi == L.nforce + L.nloop * L.w;
+  const int nblk =
+      ca ? 1 + (L.nforce / L.w - 1 + 2 * L.nloop + BD / L.w - 1) / (BD / L.w) : (L.win_hi - L.win_lo + BD) / BD;
   const int64_t grid = (int64_t)nblk * nblk * nblk * nbox;
-  hipLaunchKernelGGL((k_deposit<R, IZ, BD>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
-                     (const XT<IZ>*)xp, rhoc, off, mesh, mass);
+  if (ca)
+    hipLaunchKernelGGL((k_deposit<R, IZ, BD, true>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
+                       (const XT<IZ>*)xp, rhoc, off, mesh, mass);
+  else
+    hipLaunchKernelGGL((k_deposit<R, IZ, BD, false>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes,
+                       (const XT<IZ>*)xp, rhoc, off, mesh, mass);
   CUBE_HIP(hipGetLastError());
 }
 

@@ -147,16 +147,18 @@ using CP = typename Cplx<R>::T;
 // Pass A: one z-plane: rho (N x N reals) -> complex (N rows of N/2+1).
 template <typename R, int N, int NT>
 __global__ void __launch_bounds__(NT) k_plane_fwd(const R* __restrict__ rho, CP<R>* __restrict__ out,
-                                                  const CP<R>* __restrict__ tw) {
+                                                  const CP<R>* __restrict__ twg) {
   using C = CP<R>;
   constexpr int H = N / 2, P = H + 1;
   __shared__ C buf[N * P];
+  __shared__ C tw[N];  // W_N^e, staged once: the stages read it from LDS
   const int64_t plane = blockIdx.x;  // box * N + z
   const R* src = rho + plane * N * N;
   for (int i = threadIdx.x; i < N * H; i += NT) {
     const int y = i / H, j = i % H;
     buf[y * P + j] = CT<C>::mk(src[y * N + 2 * j], src[y * N + 2 * j + 1]);
   }
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
   __syncthreads();
   stages<-1, H, N, P, 1, false, NT, 0, 1, 2>(buf, tw);  // half-length rows
   // split: X[k] = (Z[k]+conj Z[H-k])/2 - i/2 W_N^k (Z[k]-conj Z[H-k]), k=0..H
@@ -202,32 +204,36 @@ __global__ void __launch_bounds__(NT) k_plane_fwd(const R* __restrict__ rho, CP<
   for (int i = threadIdx.x; i < N * P; i += NT) dst[i] = buf[i];
 }
 
-// Pass B: one y-slice of one box: lines along z, times G, back.
-template <typename R, int N, int NT>
+// Pass B: YB consecutive y-slices of one box: lines along z, times G, back.
+// Several slices per block make the reads and writes of each z row longer
+// contiguous runs (YB*(N/2+1) complex values).
+template <typename R, int N, int NT, int YB>
 __global__ void __launch_bounds__(NT) k_zslice(CP<R>* __restrict__ data, const float* __restrict__ G,
-                                               const CP<R>* __restrict__ tw) {
+                                               const CP<R>* __restrict__ twg) {
   using C = CP<R>;
-  constexpr int P = N / 2 + 1;
-  __shared__ C buf[N * P];
-  const int64_t box = blockIdx.x / N;
-  const int y = blockIdx.x % N;
-  C* base = data + box * (int64_t)N * N * P + (int64_t)y * P;
-  for (int i = threadIdx.x; i < N * P; i += NT) {
-    const int z = i / P, x = i % P;
+  constexpr int P = N / 2 + 1, W = YB * P;
+  __shared__ C buf[N * W];
+  __shared__ C tw[N];
+  const int64_t box = blockIdx.x / (N / YB);
+  const int y0 = (blockIdx.x % (N / YB)) * YB;
+  C* base = data + box * (int64_t)N * N * P + (int64_t)y0 * P;
+  for (int i = threadIdx.x; i < N * W; i += NT) {
+    const int z = i / W, x = i % W;
     buf[i] = base[(int64_t)z * N * P + x];
   }
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
   __syncthreads();
-  stages<-1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);
-  const float* gb = G + (int64_t)y * P;
-  for (int i = threadIdx.x; i < N * P; i += NT) {
-    const int z = i / P, x = i % P;
+  stages<-1, N, W, 1, W, true, NT, 0, 1, 1>(buf, tw);
+  const float* gb = G + (int64_t)y0 * P;
+  for (int i = threadIdx.x; i < N * W; i += NT) {
+    const int z = i / W, x = i % W;
     const R g = (R)gb[(int64_t)z * N * P + x];
     buf[i] = CT<C>::mk(buf[i].x * g, buf[i].y * g);  // cxyz*tf (kick.f90:698)
   }
   __syncthreads();
-  stages<+1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);
-  for (int i = threadIdx.x; i < N * P; i += NT) {
-    const int z = i / P, x = i % P;
+  stages<+1, N, W, 1, W, true, NT, 0, 1, 1>(buf, tw);
+  for (int i = threadIdx.x; i < N * W; i += NT) {
+    const int z = i / W, x = i % W;
     base[(int64_t)z * N * P + x] = buf[i];
   }
 }
@@ -235,13 +241,15 @@ __global__ void __launch_bounds__(NT) k_zslice(CP<R>* __restrict__ data, const f
 // Pass C: one z-plane: inverse along y, c2r along x, normalisation.
 template <typename R, int N, int NT>
 __global__ void __launch_bounds__(NT) k_plane_inv(const CP<R>* __restrict__ in, R* __restrict__ phi,
-                                                  const CP<R>* __restrict__ tw, R div3, R div1) {
+                                                  const CP<R>* __restrict__ twg, R div3, R div1) {
   using C = CP<R>;
   constexpr int H = N / 2, P = H + 1;
   __shared__ C buf[N * P];
+  __shared__ C tw[N];
   const int64_t plane = blockIdx.x;
   const C* src = in + plane * N * P;
   for (int i = threadIdx.x; i < N * P; i += NT) buf[i] = src[i];
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
   __syncthreads();
   stages<+1, N, P, 1, P, true, NT, 0, 1, 1>(buf, tw);
   // Z[k] = E + iO with E = X[k] + conj X[H-k], O = (X[k] - conj X[H-k]) W_N^-k
@@ -311,7 +319,9 @@ bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t
   }
   {
     KScope k(tm, (t + "fft_z_green").c_str(), 2 * Kb + Gk);
-    hipLaunchKernelGGL((k_zslice<R, N, NTZ>), dim3((unsigned)(nbox * N)), dim3(NTZ), 0, s, work, G, c.tw);
+    // two slices per block where their LDS fits in 64 KB
+    constexpr int YB = (N % 2 == 0 && 2 * N * (N / 2 + 1) * (int)sizeof(CP<R>) <= 65536) ? 2 : 1;
+    hipLaunchKernelGGL((k_zslice<R, N, NTZ, YB>), dim3((unsigned)(nbox * (N / YB))), dim3(NTZ), 0, s, work, G, c.tw);
     CUBE_HIP(hipGetLastError());
   }
   {

@@ -161,6 +161,16 @@ __host__ __device__ inline R f_ra(R r, R apm, R c) {
   return (R)0;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
+// with its own L2.  Kernels whose neighbouring blocks share input (halos,
+// neighbour cells) map the dispatch index i to a logical block so that each
+// XCD runs one contiguous range of logical blocks: XCD x = i % 8 takes
+// blocks [start(x), start(x) + count(x)).  A bijection on [0, n) for any n.
+__device__ inline int64_t xcd_block(int64_t i, int64_t n) {
+  const int64_t q = n >> 3, r = n & 7, x = i & 7, k = i >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // Order-preserving float max for non-negative values through integer atomics
 // (deterministic regardless of launch order).  The atomic is issued only when
 // the value exceeds what the location already holds: a running maximum

@@ -114,7 +114,13 @@ __device__ inline double pm_xpos(const PMLevel& L, int c, double frac, int shift
 // Deposits one particle's 27 TSC weights (floor f, weights w) into the
 // block's LDS accumulator.
 template <typename R, typename Acc, int BD>
-__device__ inline void deposit_particle(Acc* acc, const int f[3], const R w[3][3], const int mlo[3], const int mhi[3]) {
+__device__ inline void deposit_particle(Acc* acc, const int f[3], const R w0[3][3], const int mlo[3], const int mhi[3]) {
+  // FP32: the z weights carry the 2^32 of the fixed point; scaling by a power
+  // of two is exact, so (wx*wy)*(wz*2^32) == ((wx*wy)*wz)*2^32 bit for bit
+  R w[3][3];
+  for (int d = 0; d < 3; ++d)
+    for (int i = 0; i < 3; ++i)
+      w[d][i] = (std::is_same<R, float>::value && d == 2) ? w0[d][i] * (R)kFixF : w0[d][i];
   const bool inner = f[0] >= mlo[0] && f[0] + 2 <= mhi[0] && f[1] >= mlo[1] && f[1] + 2 <= mhi[1] &&
                      f[2] >= mlo[2] && f[2] + 2 <= mhi[2];
   for (int i3 = 0; i3 < 3; ++i3) {
@@ -129,7 +135,8 @@ __device__ inline void deposit_particle(Acc* acc, const int f[3], const R w[3][3
         const int mx = f[0] + i1;
         if (!inner && (mx < mlo[0] || mx > mhi[0])) continue;
         const R cw = (w[0][i1] * wyz) * w[2][i3];  // dx(1,i1)*dx(2,i2)*dx(3,i3)
-        if constexpr (std::is_same<R, float>::value) atomicAdd(&acc[row + mx], fix32(cw));
+        if constexpr (std::is_same<R, float>::value)
+          atomicAdd(&acc[row + mx], (unsigned long long)__float2uint_rn(cw));
         else atomicAdd(&acc[row + mx], cw);
       }
     }
@@ -609,7 +616,8 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
   const int g[3] = {IBX / w, IBY / w, IBZ / w};  // coarse cells per block per axis
   const int ncell = L.nforce / w;                // physical coarse cells of a box per axis
   const int nb[3] = {(ncell + g[0] - 1) / g[0], (ncell + g[1] - 1) / g[1], (ncell + g[2] - 1) / g[2]};
-  int64_t b = blockIdx.x;
+  // neighbouring blocks share phi halos: keep them on one XCD's L2
+  int64_t b = xcd_block(blockIdx.x, gridDim.x);
   const int bx = (int)(b % nb[0]);
   b /= nb[0];
   const int by = (int)(b % nb[1]);

@@ -1,0 +1,490 @@
+// pmdiff.hip -- differentiable PM-only force on periodic meshes and its
+// vector-Jacobian products (the C-ABI of include/cube_pm.h).
+//
+// Forward:  rho = TSC deposit, phi = c2r(G r2c(rho))/n^3, F_d = D_d phi (the
+// 4-point stencil of particle_mesh.f90:237-242), acc = TSC interpolation of F.
+// Adjoint of acc with respect to the positions, given g = dL/dacc:
+//   Fbar_d   = TSC deposit of g_d                     (transpose of interpolation)
+//   phibar   = sum_d D_d^T Fbar_d = -sum_d D_d Fbar_d (D_d is antisymmetric)
+//   rhobar   = c2r(G r2c(phibar))/n^3                 (G real and even: self-adjoint)
+//   dL/dx_p  = sum_m grad W(x_p - m) [F(m).g_p + rhobar(m)]
+// Mesh sums use integer atomics on fixed-point values: 2^-32 units for the
+// non-negative densities, and for the signed adjoint deposits a power-of-two
+// scale chosen from max|g| so that no sum can overflow.  Integer addition is
+// associative, so every output is independent of thread order.
+#include <cmath>
+#include <string>
+
+#include "../../include/cube_p3m.h"
+#include "../../include/cube_pm.h"
+#include "common.h"
+#include "fft.h"
+#include "pm.h"
+
+namespace cube {
+namespace {
+
+template <typename T>
+struct PMBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  T* ensure(size_t m) {
+    if (m > n) {
+      release();
+      const hipError_t e = hipMalloc(&p, m * sizeof(T));
+      if (e != hipSuccess) {
+        p = nullptr;
+        throw Error(CUBE_ERR_HIP, "hipMalloc of " + std::to_string(m * sizeof(T)) + " bytes failed");
+      }
+      n = m;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~PMBuf() { release(); }
+};
+
+constexpr double kFix = 4294967296.0;  // 2^32
+
+__device__ inline int wrapi(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// TSC weights and their derivatives on points i-1, i, i+1.
+__device__ inline void tsc(float x, int& i, float w[3], float dw[3]) {
+  const float fi = floorf(x + 0.5f);
+  i = (int)fi;
+  const float d = x - fi;
+  const float a = 0.5f - d, b = 0.5f + d;
+  w[0] = 0.5f * a * a;
+  w[1] = 0.75f - d * d;
+  w[2] = 0.5f * b * b;
+  dw[0] = -a;
+  dw[1] = -2.0f * d;
+  dw[2] = b;
+}
+
+// CIC weights on points i, i+1 (cell i covers [i, i+1)).
+__device__ inline void cic(float y, int& i, float w[2]) {
+  const float fi = floorf(y);
+  i = (int)fi;
+  const float d = y - fi;
+  w[0] = 1.0f - d;
+  w[1] = d;
+}
+
+__global__ void k_zero_u32(unsigned int* a, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+__global__ void k_zero_u64(unsigned long long* a, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+// rho += W (fixed point 2^-32); one thread per particle.
+__global__ void k_tsc_deposit(const float* __restrict__ x, int64_t np, int64_t ntot, int n,
+                              unsigned long long* __restrict__ acc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int64_t b = p / np, n3 = (int64_t)n * n * n;
+  int i[3];
+  float w[3][3], dw[3][3];
+  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  unsigned long long* a = acc + b * n3;
+  for (int k3 = 0; k3 < 3; ++k3) {
+    const int z = wrapi(i[2] - 1 + k3, n);
+    for (int k2 = 0; k2 < 3; ++k2) {
+      const int y = wrapi(i[1] - 1 + k2, n);
+      const double wyz = (double)(w[1][k2] * w[2][k3]);
+      for (int k1 = 0; k1 < 3; ++k1) {
+        const int xx = wrapi(i[0] - 1 + k1, n);
+        atomicAdd(&a[((int64_t)z * n + y) * n + xx], (unsigned long long)llrint((double)w[0][k1] * wyz * kFix));
+      }
+    }
+  }
+}
+
+// out = acc * 2^-32 (as float), acc cleared for the next use.
+__global__ void k_fix_to_float(unsigned long long* __restrict__ acc, int64_t n, float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = (float)((double)acc[t] * (1.0 / kFix));
+  acc[t] = 0;
+}
+
+// F[b][d] = D_d phi[b] (periodic).
+__global__ void k_fd_force(const float* __restrict__ phi, int B, int n, float* __restrict__ F) {
+  const int64_t n3 = (int64_t)n * n * n;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * n3) return;
+  const int64_t b = t / n3, m = t % n3;
+  const int x = (int)(m % n), y = (int)((m / n) % n), z = (int)(m / ((int64_t)n * n));
+  const float* ph = phi + b * n3;
+  const FDWeights<float> W;
+  const int s[4] = {-2, -1, 1, 2};
+  const float c[4] = {W.w0, W.w1, W.w2, W.w3};
+  float f[3] = {0, 0, 0};
+  for (int q = 0; q < 4; ++q) {
+    f[0] = f[0] + ph[((int64_t)z * n + y) * n + wrapi(x + s[q], n)] * c[q];
+    f[1] = f[1] + ph[((int64_t)z * n + wrapi(y + s[q], n)) * n + x] * c[q];
+    f[2] = f[2] + ph[((int64_t)wrapi(z + s[q], n) * n + y) * n + x] * c[q];
+  }
+  for (int d = 0; d < 3; ++d) F[(b * 3 + d) * n3 + m] = f[d];
+}
+
+// phibar[b] = -sum_d D_d Fbar[b][d] (transpose of k_fd_force).
+__global__ void k_fd_adjoint(const float* __restrict__ Fb, int B, int n, float* __restrict__ phib) {
+  const int64_t n3 = (int64_t)n * n * n;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * n3) return;
+  const int64_t b = t / n3, m = t % n3;
+  const int x = (int)(m % n), y = (int)((m / n) % n), z = (int)(m / ((int64_t)n * n));
+  const FDWeights<float> W;
+  const int s[4] = {-2, -1, 1, 2};
+  const float c[4] = {W.w0, W.w1, W.w2, W.w3};
+  const float* f0 = Fb + (b * 3 + 0) * n3;
+  const float* f1 = Fb + (b * 3 + 1) * n3;
+  const float* f2 = Fb + (b * 3 + 2) * n3;
+  float v = 0;
+  for (int q = 0; q < 4; ++q) {
+    v = v + f0[((int64_t)z * n + y) * n + wrapi(x + s[q], n)] * c[q];
+    v = v + f1[((int64_t)z * n + wrapi(y + s[q], n)) * n + x] * c[q];
+    v = v + f2[((int64_t)wrapi(z + s[q], n) * n + y) * n + x] * c[q];
+  }
+  phib[t] = -v;
+}
+
+// acc_p = sum W F.
+__global__ void k_tsc_interp(const float* __restrict__ x, int64_t np, int64_t ntot, int n,
+                             const float* __restrict__ F, float* __restrict__ acc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int64_t b = p / np, n3 = (int64_t)n * n * n;
+  int i[3];
+  float w[3][3], dw[3][3];
+  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  const float* Fb = F + b * 3 * n3;
+  float a[3] = {0, 0, 0};
+  for (int k3 = 0; k3 < 3; ++k3) {
+    const int z = wrapi(i[2] - 1 + k3, n);
+    for (int k2 = 0; k2 < 3; ++k2) {
+      const int y = wrapi(i[1] - 1 + k2, n);
+      const float wyz = w[1][k2] * w[2][k3];
+      for (int k1 = 0; k1 < 3; ++k1) {
+        const int64_t m = ((int64_t)z * n + y) * n + wrapi(i[0] - 1 + k1, n);
+        const float wt = w[0][k1] * wyz;
+        for (int d = 0; d < 3; ++d) a[d] = fmaf(wt, Fb[d * n3 + m], a[d]);
+      }
+    }
+  }
+  for (int d = 0; d < 3; ++d) acc[3 * p + d] = a[d];
+}
+
+// gmax = max |g| (float bits; non-negative floats order as integers).
+__global__ void k_absmax(const float* __restrict__ g, int64_t n, unsigned int* __restrict__ gmax) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float v = t < n ? fabsf(g[t]) : 0.0f;
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(gmax, __float_as_uint(v));
+}
+
+// Power-of-two scale S with |sum| * S < 2^62 for every mesh sum of
+// contributions bounded by ntot * max|g| (each particle adds W <= 1 per point).
+__device__ inline double fix_scale(const unsigned int* gmax, int64_t ntot) {
+  const double m = (double)__uint_as_float(*gmax) * 3.0 * (double)ntot;
+  if (!(m > 0)) return 1.0;
+  return ldexp(1.0, 61 - ilogb(m) - 1);
+}
+
+// Fbar[b][d] += W g_d in fixed point of scale S.
+__global__ void k_tsc_deposit_vec(const float* __restrict__ x, const float* __restrict__ g, int64_t np, int64_t ntot,
+                                  int n, const unsigned int* __restrict__ gmax, unsigned long long* __restrict__ acc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const double S = fix_scale(gmax, ntot);
+  const int64_t b = p / np, n3 = (int64_t)n * n * n;
+  int i[3];
+  float w[3][3], dw[3][3];
+  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  const double gs[3] = {(double)g[3 * p] * S, (double)g[3 * p + 1] * S, (double)g[3 * p + 2] * S};
+  unsigned long long* a = acc + b * 3 * n3;
+  for (int k3 = 0; k3 < 3; ++k3) {
+    const int z = wrapi(i[2] - 1 + k3, n);
+    for (int k2 = 0; k2 < 3; ++k2) {
+      const int y = wrapi(i[1] - 1 + k2, n);
+      const double wyz = (double)(w[1][k2] * w[2][k3]);
+      for (int k1 = 0; k1 < 3; ++k1) {
+        const int64_t m = ((int64_t)z * n + y) * n + wrapi(i[0] - 1 + k1, n);
+        const double wt = (double)w[0][k1] * wyz;
+        for (int d = 0; d < 3; ++d)
+          if (gs[d] != 0.0) atomicAdd(&a[d * n3 + m], (unsigned long long)llrint(wt * gs[d]));
+      }
+    }
+  }
+}
+
+// out = (signed) acc / S, acc cleared.
+__global__ void k_sfix_to_float(unsigned long long* __restrict__ acc, int64_t n, const unsigned int* __restrict__ gmax,
+                                int64_t ntot, float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double S = fix_scale(gmax, ntot);
+  out[t] = (float)((double)(long long)acc[t] / S);
+  acc[t] = 0;
+}
+
+// g_x = sum grad W (F.g + rhobar).
+__global__ void k_tsc_vjp_gather(const float* __restrict__ x, int64_t np, int64_t ntot, int n,
+                                 const float* __restrict__ F, const float* __restrict__ g,
+                                 const float* __restrict__ rhob, float* __restrict__ gx) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int64_t b = p / np, n3 = (int64_t)n * n * n;
+  int i[3];
+  float w[3][3], dw[3][3];
+  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  const float gp[3] = {g[3 * p], g[3 * p + 1], g[3 * p + 2]};
+  const float* Fb = F + b * 3 * n3;
+  const float* rb = rhob + b * n3;
+  float o[3] = {0, 0, 0};
+  for (int k3 = 0; k3 < 3; ++k3) {
+    const int z = wrapi(i[2] - 1 + k3, n);
+    for (int k2 = 0; k2 < 3; ++k2) {
+      const int y = wrapi(i[1] - 1 + k2, n);
+      for (int k1 = 0; k1 < 3; ++k1) {
+        const int64_t m = ((int64_t)z * n + y) * n + wrapi(i[0] - 1 + k1, n);
+        const float s = fmaf(Fb[m], gp[0], fmaf(Fb[n3 + m], gp[1], fmaf(Fb[2 * n3 + m], gp[2], rb[m])));
+        o[0] = fmaf(dw[0][k1] * w[1][k2] * w[2][k3], s, o[0]);
+        o[1] = fmaf(w[0][k1] * dw[1][k2] * w[2][k3], s, o[1]);
+        o[2] = fmaf(w[0][k1] * w[1][k2] * dw[2][k3], s, o[2]);
+      }
+    }
+  }
+  for (int d = 0; d < 3; ++d) gx[3 * p + d] = o[d];
+}
+
+// CIC density of x*scale on nw^3 (fixed point 2^-32).
+__global__ void k_cic_deposit(const float* __restrict__ x, int64_t np, int64_t ntot, int nw, float scale,
+                              unsigned long long* __restrict__ acc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int64_t b = p / np, n3 = (int64_t)nw * nw * nw;
+  int i[3];
+  float w[3][2];
+  for (int d = 0; d < 3; ++d) cic(x[3 * p + d] * scale, i[d], w[d]);
+  unsigned long long* a = acc + b * n3;
+  for (int k3 = 0; k3 < 2; ++k3)
+    for (int k2 = 0; k2 < 2; ++k2)
+      for (int k1 = 0; k1 < 2; ++k1) {
+        const int64_t m = ((int64_t)wrapi(i[2] + k3, nw) * nw + wrapi(i[1] + k2, nw)) * nw + wrapi(i[0] + k1, nw);
+        const double wt = (double)w[0][k1] * (double)w[1][k2] * (double)w[2][k3];
+        atomicAdd(&a[m], (unsigned long long)llrint(wt * kFix));
+      }
+}
+
+__global__ void k_cic_vjp(const float* __restrict__ x, int64_t np, int64_t ntot, int nw, float scale,
+                          const float* __restrict__ grho, float* __restrict__ gx) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int64_t b = p / np, n3 = (int64_t)nw * nw * nw;
+  int i[3];
+  float w[3][2];
+  for (int d = 0; d < 3; ++d) cic(x[3 * p + d] * scale, i[d], w[d]);
+  const float dw[2] = {-1.0f, 1.0f};
+  const float* gr = grho + b * n3;
+  float o[3] = {0, 0, 0};
+  for (int k3 = 0; k3 < 2; ++k3)
+    for (int k2 = 0; k2 < 2; ++k2)
+      for (int k1 = 0; k1 < 2; ++k1) {
+        const float v =
+            gr[((int64_t)wrapi(i[2] + k3, nw) * nw + wrapi(i[1] + k2, nw)) * nw + wrapi(i[0] + k1, nw)];
+        o[0] = fmaf(dw[k1] * w[1][k2] * w[2][k3], v, o[0]);
+        o[1] = fmaf(w[0][k1] * dw[k2] * w[2][k3], v, o[1]);
+        o[2] = fmaf(w[0][k1] * w[1][k2] * dw[k3], v, o[2]);
+      }
+  for (int d = 0; d < 3; ++d) gx[3 * p + d] = o[d] * scale;
+}
+
+inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+}  // namespace cube
+
+using namespace cube;
+
+struct cube_pm_ctx {
+  int device = 0;
+  cube_pm_config cfg{};
+  int64_t n3 = 0;
+  FFTCube<float> fc;
+  PMBuf<float> G, mesh, Fs, Fbar;
+  PMBuf<float2> work;
+  PMBuf<unsigned long long> acc;   // B*3*n^3, kept zero between calls
+  PMBuf<unsigned long long> accw;  // B*nw^3 for the CIC density, kept zero
+  PMBuf<unsigned int> gmax;
+  int nw_alloc = 0;
+  std::string err;
+
+  void check_np(int64_t np) const {
+    if (np <= 0) throw Error(CUBE_ERR_ARG, "np must be positive");
+  }
+  // rho -> phi -> F (B*3*n^3) at positions x
+  void force_meshes(const float* x, int64_t np, float* F, hipStream_t s) {
+    const int n = cfg.n, B = cfg.batch;
+    const int64_t ntot = np * B;
+    hipLaunchKernelGGL(k_tsc_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, acc.p);
+    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * n3)), dim3(256), 0, s, acc.p, B * n3, mesh.p);
+    fft_convolve<float>(fc, mesh.p, work.p, G.p, B, (float)n3, 0.0f, s);
+    hipLaunchKernelGGL(k_fd_force, dim3(nblk(B * n3)), dim3(256), 0, s, mesh.p, B, n, F);
+    CUBE_HIP(hipGetLastError());
+  }
+};
+
+namespace {
+template <class Fn>
+int pm_guarded(cube_pm_ctx* ctx, Fn&& f) {
+  if (!ctx) return CUBE_ERR_ARG;
+  try {
+    CUBE_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return CUBE_OK;
+  } catch (const Error& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return CUBE_ERR_HIP;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int cube_pm_create(const cube_pm_config* cfg, cube_pm_ctx** out) {
+  if (!cfg || !out) return CUBE_ERR_ARG;
+  *out = nullptr;
+  if (cfg->n < 8 || cfg->batch < 1 || !(cfg->apm_soft > 0)) return CUBE_ERR_ARG;
+  auto* c = new cube_pm_ctx();
+  c->cfg = *cfg;
+  const int rc = pm_guarded(c, [&] {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(CUBE_ERR_NO_DEVICE, "no HIP device");
+    CUBE_HIP(hipGetDevice(&c->device));
+    const int n = cfg->n, B = cfg->batch;
+    c->n3 = (int64_t)n * n * n;
+    c->fc.init(n);
+    const int64_t nk = (int64_t)n * n * (n / 2 + 1);
+    c->G.ensure((size_t)nk);
+    green_device(n, cfg->apm_soft, 0.0f, 1.0f, 1.0, c->G.p, nullptr);
+    c->mesh.ensure((size_t)(B * c->n3));
+    c->Fs.ensure((size_t)(3 * B * c->n3));
+    c->Fbar.ensure((size_t)(3 * B * c->n3));
+    c->work.ensure((size_t)(B * nk));
+    c->acc.ensure((size_t)(3 * B * c->n3));
+    c->gmax.ensure(1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(nblk(3 * B * c->n3)), dim3(256), 0, nullptr, c->acc.p, 3 * B * c->n3);
+    CUBE_HIP(hipGetLastError());
+    CUBE_HIP(hipDeviceSynchronize());
+  });
+  if (rc != CUBE_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return CUBE_OK;
+}
+
+int cube_pm_destroy(cube_pm_ctx* ctx) {
+  if (!ctx) return CUBE_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  ctx->fc.release();
+  delete ctx;
+  return CUBE_OK;
+}
+
+const char* cube_pm_last_error(const cube_pm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cube_pm_green(cube_pm_ctx* ctx, float* G_host) {
+  return pm_guarded(ctx, [&] {
+    if (!G_host) throw Error(CUBE_ERR_ARG, "null output");
+    const int n = ctx->cfg.n;
+    CUBE_HIP(hipMemcpy(G_host, ctx->G.p, sizeof(float) * (size_t)n * n * (n / 2 + 1), hipMemcpyDeviceToHost));
+  });
+}
+
+int cube_pm_force(cube_pm_ctx* ctx, const float* x, int64_t np, float* F, float* acc, void* stream) {
+  return pm_guarded(ctx, [&] {
+    ctx->check_np(np);
+    if (!x || !acc) throw Error(CUBE_ERR_ARG, "null pointer");
+    const hipStream_t s = (hipStream_t)stream;
+    float* Fm = F ? F : ctx->Fs.p;
+    ctx->force_meshes(x, np, Fm, s);
+    const int64_t ntot = np * ctx->cfg.batch;
+    hipLaunchKernelGGL(k_tsc_interp, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, ctx->cfg.n, Fm, acc);
+    CUBE_HIP(hipGetLastError());
+  });
+}
+
+int cube_pm_force_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, const float* F, const float* g_acc, float* g_x,
+                      void* stream) {
+  return pm_guarded(ctx, [&] {
+    ctx->check_np(np);
+    if (!x || !F || !g_acc || !g_x) throw Error(CUBE_ERR_ARG, "null pointer");
+    const hipStream_t s = (hipStream_t)stream;
+    const int n = ctx->cfg.n, B = ctx->cfg.batch;
+    const int64_t ntot = np * B, n3 = ctx->n3;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctx->gmax.p, 1);
+    hipLaunchKernelGGL(k_absmax, dim3(nblk(3 * ntot)), dim3(256), 0, s, g_acc, 3 * ntot, ctx->gmax.p);
+    hipLaunchKernelGGL(k_tsc_deposit_vec, dim3(nblk(ntot)), dim3(256), 0, s, x, g_acc, np, ntot, n, ctx->gmax.p,
+                       ctx->acc.p);
+    hipLaunchKernelGGL(k_sfix_to_float, dim3(nblk(3 * B * n3)), dim3(256), 0, s, ctx->acc.p, 3 * B * n3, ctx->gmax.p,
+                       ntot, ctx->Fbar.p);
+    hipLaunchKernelGGL(k_fd_adjoint, dim3(nblk(B * n3)), dim3(256), 0, s, ctx->Fbar.p, B, n, ctx->mesh.p);
+    fft_convolve<float>(ctx->fc, ctx->mesh.p, ctx->work.p, ctx->G.p, B, (float)n3, 0.0f, s);
+    hipLaunchKernelGGL(k_tsc_vjp_gather, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, F, g_acc, ctx->mesh.p,
+                       g_x);
+    CUBE_HIP(hipGetLastError());
+  });
+}
+
+int cube_pm_cic(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float scale, float* rho, void* stream) {
+  return pm_guarded(ctx, [&] {
+    ctx->check_np(np);
+    if (!x || !rho || nw < 2) throw Error(CUBE_ERR_ARG, "bad argument");
+    const hipStream_t s = (hipStream_t)stream;
+    const int B = ctx->cfg.batch;
+    const int64_t ntot = np * B, nw3 = (int64_t)nw * nw * nw;
+    if (ctx->nw_alloc != nw) {
+      ctx->accw.release();
+      ctx->accw.ensure((size_t)(B * nw3));
+      hipLaunchKernelGGL(k_zero_u64, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3);
+      ctx->nw_alloc = nw;
+    }
+    hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->accw.p);
+    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3, rho);
+    CUBE_HIP(hipGetLastError());
+  });
+}
+
+int cube_pm_cic_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float scale, const float* g_rho,
+                    float* g_x, void* stream) {
+  return pm_guarded(ctx, [&] {
+    ctx->check_np(np);
+    if (!x || !g_rho || !g_x || nw < 2) throw Error(CUBE_ERR_ARG, "bad argument");
+    const hipStream_t s = (hipStream_t)stream;
+    const int64_t ntot = np * ctx->cfg.batch;
+    hipLaunchKernelGGL(k_cic_vjp, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, g_rho, g_x);
+    CUBE_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

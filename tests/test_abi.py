@@ -63,3 +63,14 @@ def test_fortran_driver_links_the_library():
 
     r = subprocess.run([DRIVER, "--version"], capture_output=True, text=True, check=True)
     assert "gfx950" in r.stdout
+
+
+def test_pm_header_lists_all_bound_functions_and_library_exports_them():
+    from cube_nu_amd.pmdiff import PM_ABI_FUNCTIONS
+
+    src = open(os.path.join(ROOT, "include", "cube_pm.h")).read()
+    names = sorted(set(re.findall(r"^(?:int|const char\*)\s+(cube_pm_\w+)\(", src, re.M)))
+    assert names == sorted(PM_ABI_FUNCTIONS)
+    lib = load_library()
+    for name in names:
+        assert hasattr(lib, name), name

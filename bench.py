@@ -150,6 +150,78 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
 
 
+def pm_forward_adjoint(reps=2):
+    """PM-only forward and forward+adjoint time (the differentiable path):
+    a batch of 4 seeds of 128^3 particles on 256^3 periodic meshes, box
+    256 Mpc/h, Zel'dovich from white noise, 10 KDK steps from a = 0.1 to 1,
+    loss = MSE between the final CIC densities on 128^3 and the CIC density
+    of a fifth seed; gradient with respect to the 4 x 128^3 noise modes."""
+    import torch
+    from cube_nu_amd import pmdiff
+
+    B, ng, n, nw, nsteps, box = 4, 128, 256, 128, 10, 256.0
+    k1, d, k2 = pmdiff.kdk_coefficients(0.1, 1.0, nsteps)
+    rho_mean = ng ** 3 / n ** 3
+    k1 = [c / (4 * 3.141592653589793 * rho_mean) for c in k1]
+    k2 = [c / (4 * 3.141592653589793 * rho_mean) for c in k2]
+    amp = 0.5
+    pm = pmdiff.PMForce(n, B, 1.0)
+    pm1 = pmdiff.PMForce(n, 1, 1.0)
+    gen = torch.Generator(device="cuda").manual_seed(20)
+    noise = torch.randn(B, ng, ng, ng, device="cuda", generator=gen)
+    z5 = torch.randn(1, ng, ng, ng, device="cuda", generator=gen)
+    with torch.no_grad():
+        x, v = pmdiff.zeldovich(z5, box, n, amp)
+        x, v = pmdiff.kdk(x, v, pm1, k1, d, k2)
+        target = pmdiff.cic_density(x, pm1, nw, nw / n)
+
+    def loss_of(z):
+        x, v = pmdiff.zeldovich(z, box, n, amp)
+        x, v = pmdiff.kdk(x, v, pm, k1, d, k2)
+        rho = pmdiff.cic_density(x, pm, nw, nw / n)
+        return ((rho.double() - target.double()) ** 2).mean()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    def fwd():
+        with torch.no_grad():
+            loss_of(noise)
+
+    def fwd_adj():
+        z = noise.clone().requires_grad_(True)
+        loss_of(z).backward()
+        return z.grad
+
+    t_f = timed(fwd)
+    t_fa = timed(fwd_adj)
+    # directional derivative against centred finite differences along a
+    # direction confined to a 16^3 block of one seed's noise (FP32 forward:
+    # the error has a truncation part ~eps^2 and a rounding part ~1/eps, so
+    # several step sizes are reported)
+    g = fwd_adj()
+    h = torch.zeros_like(noise)
+    h[0, :16, :16, :16] = torch.randn((16, 16, 16), device="cuda", generator=gen)
+    gd = float((g.double() * h.double()).sum())
+    fd_err = {}
+    with torch.no_grad():
+        for eps in (0.3, 0.1, 0.03, 0.01):
+            fd = float((loss_of(noise + eps * h) - loss_of(noise - eps * h)) / (2 * eps))
+            fd_err[f"{eps:g}"] = abs(gd - fd) / abs(fd)
+    pm.close()
+    pm1.close()
+    return {"config": "4 seeds x 128^3 particles, 256^3 mesh, box 256 Mpc/h, 10 KDK steps a=0.1->1, "
+                      "MSE of 128^3 CIC density vs a fifth seed, gradient w.r.t. the noise modes",
+            "forward_ms": t_f, "forward_adjoint_ms": t_fa, "adjoint_over_forward": t_fa / t_f,
+            "fd_directional_rel_err": fd_err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +233,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
     ap.add_argument("--cpu-sample-ng", type=int, default=384, help="0 = skip the CPU baseline")
+    ap.add_argument("--pm-adjoint", type=int, default=1, help="0 = skip the PM forward+adjoint timing")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -185,6 +258,12 @@ def main():
                 return {"error": str(e)}
         fp64 = form(False)
         fp64_plain = form(True)
+    pm_adj = None
+    if args.pm_adjoint:
+        try:
+            pm_adj = pm_forward_adjoint()
+        except Exception as e:  # reported, not hidden
+            pm_adj = {"error": str(e)}
     cpu = None
     if args.cpu_sample_ng > 0:
         try:
@@ -200,7 +279,7 @@ def main():
                                "(drift + buffer + PM1/PM2/PM3/PP kick), int16 coordinates",
                    "ng": args.ng, "izipx": 2, "dt": r32["dt"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64_form": fp64, "fp64_plain_per_tile": fp64_plain,
-        "phase_ms_per_step": phases,
+        "phase_ms_per_step": phases, "pm_forward_adjoint": pm_adj,
     }
     print(json.dumps(line), flush=True)
 

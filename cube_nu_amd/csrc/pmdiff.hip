@@ -9,9 +9,11 @@
 //   rhobar   = c2r(G r2c(phibar))/n^3                 (G real and even: self-adjoint)
 //   dL/dx_p  = sum_m grad W(x_p - m) [F(m).g_p + rhobar(m)]
 // Mesh sums use integer atomics on fixed-point values: 2^-32 units for the
-// non-negative densities, and for the signed adjoint deposits a power-of-two
-// scale chosen from max|g| so that no sum can overflow.  Integer addition is
+// TSC density, 2^-31 for the CIC density, and for the signed adjoint
+// deposits a power-of-two scale chosen from max|g| so that every
+// contribution is a 32-bit integer and no sum can overflow.  Integer addition is
 // associative, so every output is independent of thread order.
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -102,20 +104,20 @@ __global__ void k_tsc_deposit(const float* __restrict__ x, int64_t np, int64_t n
     const int z = wrapi(i[2] - 1 + k3, n);
     for (int k2 = 0; k2 < 3; ++k2) {
       const int y = wrapi(i[1] - 1 + k2, n);
-      const double wyz = (double)(w[1][k2] * w[2][k3]);
+      const float wyz32 = (w[1][k2] * w[2][k3]) * 4294967296.0f;  // weights <= 0.42: fits a u32
       for (int k1 = 0; k1 < 3; ++k1) {
         const int xx = wrapi(i[0] - 1 + k1, n);
-        atomicAdd(&a[((int64_t)z * n + y) * n + xx], (unsigned long long)llrint((double)w[0][k1] * wyz * kFix));
+        atomicAdd(&a[((int64_t)z * n + y) * n + xx], (unsigned long long)__float2uint_rn(w[0][k1] * wyz32));
       }
     }
   }
 }
 
-// out = acc * 2^-32 (as float), acc cleared for the next use.
-__global__ void k_fix_to_float(unsigned long long* __restrict__ acc, int64_t n, float* __restrict__ out) {
+// out = acc * unit (as float), acc cleared for the next use.
+__global__ void k_fix_to_float(unsigned long long* __restrict__ acc, int64_t n, double unit, float* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  out[t] = (float)((double)acc[t] * (1.0 / kFix));
+  out[t] = (float)((double)acc[t] * unit);
   acc[t] = 0;
 }
 
@@ -188,19 +190,30 @@ __global__ void k_tsc_interp(const float* __restrict__ x, int64_t np, int64_t nt
 }
 
 // gmax = max |g| (float bits; non-negative floats order as integers).
-__global__ void k_absmax(const float* __restrict__ g, int64_t n, unsigned int* __restrict__ gmax) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float v = t < n ? fabsf(g[t]) : 0.0f;
+// Grid-stride over a fixed grid, then one atomic per block.
+constexpr int kMaxBlocks = 1024;
+__global__ void __launch_bounds__(256) k_absmax(const float* __restrict__ g, int64_t n, unsigned int* __restrict__ gmax) {
+  __shared__ float red[4];
+  float v = 0.0f;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    v = fmaxf(v, fabsf(g[t]));
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
-  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(gmax, __float_as_uint(v));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > 0) atomicMax(gmax, __float_as_uint(m));
+  }
 }
 
-// Power-of-two scale S with |sum| * S < 2^62 for every mesh sum of
-// contributions bounded by ntot * max|g| (each particle adds W <= 1 per point).
-__device__ inline double fix_scale(const unsigned int* gmax, int64_t ntot) {
-  const double m = (double)__uint_as_float(*gmax) * 3.0 * (double)ntot;
+// Power-of-two scale S with max|g| * S in [2^30, 2^31): every contribution
+// W g S (W <= 0.42) rounds to a 32-bit integer, and a mesh sum of at most
+// 27 * ntot of them stays far inside 64 bits.  The rounding is 2^-31 of
+// max|g| per contribution.
+__device__ inline double fix_scale(const unsigned int* gmax) {
+  const float m = __uint_as_float(*gmax);
   if (!(m > 0)) return 1.0;
-  return ldexp(1.0, 61 - ilogb(m) - 1);
+  return ldexp(1.0, 30 - ilogbf(m));
 }
 
 // Fbar[b][d] += W g_d in fixed point of scale S.
@@ -208,23 +221,25 @@ __global__ void k_tsc_deposit_vec(const float* __restrict__ x, const float* __re
                                   int n, const unsigned int* __restrict__ gmax, unsigned long long* __restrict__ acc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
-  const double S = fix_scale(gmax, ntot);
+  const double S = fix_scale(gmax);
   const int64_t b = p / np, n3 = (int64_t)n * n * n;
   int i[3];
   float w[3][3], dw[3][3];
   for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
-  const double gs[3] = {(double)g[3 * p] * S, (double)g[3 * p + 1] * S, (double)g[3 * p + 2] * S};
+  const float gs[3] = {(float)(g[3 * p] * S), (float)(g[3 * p + 1] * S), (float)(g[3 * p + 2] * S)};
   unsigned long long* a = acc + b * 3 * n3;
   for (int k3 = 0; k3 < 3; ++k3) {
     const int z = wrapi(i[2] - 1 + k3, n);
     for (int k2 = 0; k2 < 3; ++k2) {
       const int y = wrapi(i[1] - 1 + k2, n);
-      const double wyz = (double)(w[1][k2] * w[2][k3]);
+      const float wyz = w[1][k2] * w[2][k3];
       for (int k1 = 0; k1 < 3; ++k1) {
         const int64_t m = ((int64_t)z * n + y) * n + wrapi(i[0] - 1 + k1, n);
-        const double wt = (double)w[0][k1] * wyz;
-        for (int d = 0; d < 3; ++d)
-          if (gs[d] != 0.0) atomicAdd(&a[d * n3 + m], (unsigned long long)llrint(wt * gs[d]));
+        const float wt = w[0][k1] * wyz;
+        for (int d = 0; d < 3; ++d) {
+          const int c = __float2int_rn(wt * gs[d]);
+          if (c != 0) atomicAdd(&a[d * n3 + m], (unsigned long long)(long long)c);
+        }
       }
     }
   }
@@ -232,10 +247,10 @@ __global__ void k_tsc_deposit_vec(const float* __restrict__ x, const float* __re
 
 // out = (signed) acc / S, acc cleared.
 __global__ void k_sfix_to_float(unsigned long long* __restrict__ acc, int64_t n, const unsigned int* __restrict__ gmax,
-                                int64_t ntot, float* __restrict__ out) {
+                                float* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double S = fix_scale(gmax, ntot);
+  const double S = fix_scale(gmax);
   out[t] = (float)((double)(long long)acc[t] / S);
   acc[t] = 0;
 }
@@ -284,8 +299,8 @@ __global__ void k_cic_deposit(const float* __restrict__ x, int64_t np, int64_t n
     for (int k2 = 0; k2 < 2; ++k2)
       for (int k1 = 0; k1 < 2; ++k1) {
         const int64_t m = ((int64_t)wrapi(i[2] + k3, nw) * nw + wrapi(i[1] + k2, nw)) * nw + wrapi(i[0] + k1, nw);
-        const double wt = (double)w[0][k1] * (double)w[1][k2] * (double)w[2][k3];
-        atomicAdd(&a[m], (unsigned long long)llrint(wt * kFix));
+        const float wt = w[0][k1] * w[1][k2] * w[2][k3];  // <= 1: fits a u32 at 2^31
+        atomicAdd(&a[m], (unsigned long long)__float2uint_rn(wt * 2147483648.0f));
       }
 }
 
@@ -340,7 +355,7 @@ struct cube_pm_ctx {
     const int n = cfg.n, B = cfg.batch;
     const int64_t ntot = np * B;
     hipLaunchKernelGGL(k_tsc_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, acc.p);
-    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * n3)), dim3(256), 0, s, acc.p, B * n3, mesh.p);
+    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * n3)), dim3(256), 0, s, acc.p, B * n3, 1.0 / kFix, mesh.p);
     fft_convolve<float>(fc, mesh.p, work.p, G.p, B, (float)n3, 0.0f, s);
     hipLaunchKernelGGL(k_fd_force, dim3(nblk(B * n3)), dim3(256), 0, s, mesh.p, B, n, F);
     CUBE_HIP(hipGetLastError());
@@ -443,11 +458,12 @@ int cube_pm_force_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, const float*
     const int n = ctx->cfg.n, B = ctx->cfg.batch;
     const int64_t ntot = np * B, n3 = ctx->n3;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctx->gmax.p, 1);
-    hipLaunchKernelGGL(k_absmax, dim3(nblk(3 * ntot)), dim3(256), 0, s, g_acc, 3 * ntot, ctx->gmax.p);
+    hipLaunchKernelGGL(k_absmax, dim3(std::min<int64_t>(nblk(3 * ntot), kMaxBlocks)), dim3(256), 0, s, g_acc,
+                       3 * ntot, ctx->gmax.p);
     hipLaunchKernelGGL(k_tsc_deposit_vec, dim3(nblk(ntot)), dim3(256), 0, s, x, g_acc, np, ntot, n, ctx->gmax.p,
                        ctx->acc.p);
     hipLaunchKernelGGL(k_sfix_to_float, dim3(nblk(3 * B * n3)), dim3(256), 0, s, ctx->acc.p, 3 * B * n3, ctx->gmax.p,
-                       ntot, ctx->Fbar.p);
+                       ctx->Fbar.p);
     hipLaunchKernelGGL(k_fd_adjoint, dim3(nblk(B * n3)), dim3(256), 0, s, ctx->Fbar.p, B, n, ctx->mesh.p);
     fft_convolve<float>(ctx->fc, ctx->mesh.p, ctx->work.p, ctx->G.p, B, (float)n3, 0.0f, s);
     hipLaunchKernelGGL(k_tsc_vjp_gather, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, F, g_acc, ctx->mesh.p,
@@ -470,7 +486,8 @@ int cube_pm_cic(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float 
       ctx->nw_alloc = nw;
     }
     hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->accw.p);
-    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3, rho);
+    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3, 1.0 / 2147483648.0,
+                       rho);
     CUBE_HIP(hipGetLastError());
   });
 }

@@ -240,6 +240,8 @@ def main():
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
         return
+    if args.pm_adjoint:
+        import torch  # noqa: F401  (imported first: P3M then lets torch's runtime initialise the device)
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this benchmark runs the one-image step on one GPU only; "
                                    "a multi-GPU (nn>1) decomposition is not implemented"}))

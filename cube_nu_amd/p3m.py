@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -104,6 +105,19 @@ ABI_FUNCTIONS = (
 )
 
 _lib = None
+
+
+def torch_runtime_first() -> None:
+    """The PyTorch-ROCm wheel carries its own HIP runtime; when both runtimes
+    live in one process, torch's must initialise the device first or
+    torch.cuda stays unavailable.  Called before the library's first device
+    call whenever torch has been imported (no effect otherwise)."""
+    t = sys.modules.get("torch")
+    if t is not None:
+        try:
+            t.cuda.init()
+        except Exception:
+            pass
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -232,6 +246,7 @@ class P3M:
                      int(pp), int(pp_corr), f_nu, mass_nu, image_buffer, tile_buffer, workspace_bytes)
         self.cfg = cfg
         ctx = C.c_void_p()
+        torch_runtime_first()
         rc = self.lib.cube_p3m_create(C.byref(cfg), C.byref(ctx))
         if rc != CUBE_OK:
             raise CubeError(rc, "cube_p3m_create failed")

@@ -596,16 +596,17 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
 // every coarse cell its cells touch.  The staging, FD force, f2max and the
 // 27-point gather are those of k_interp.
 constexpr int IBX = 8, IBY = 8, IBZ = 16;
+constexpr int IT = 256;  // threads per block
 
 template <typename R, int IZ>
-__global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __restrict__ boxes,
+__global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __restrict__ boxes,
                                                       const XT<IZ>* __restrict__ xp, float* __restrict__ vp,
                                                       const int32_t* __restrict__ rhoc,
                                                       const int64_t* __restrict__ off, const R* __restrict__ phi,
                                                       float a_mid, float dt, PMOut* __restrict__ out) {
   constexpr int FX = IBX + 2, FY = IBY + 2, FZ = IBZ + 2, FN = FX * FY * FZ;
   constexpr int PX = IBX + 6, PY = IBY + 6, PZ = IBZ + 6, PN = PX * PY * PZ;
-  constexpr int NF = (FN + 255) / 256, NPH = (PN + 255) / 256;
+  constexpr int NF = (FN + IT - 1) / IT, NPH = (PN + IT - 1) / IT;
   constexpr int LDSN = PN > 3 * FN ? PN : 3 * FN;
   constexpr int MAXC = 16;
   __shared__ R lds[LDSN];
@@ -638,7 +639,7 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
   R ph[NPH];
 #pragma unroll
   for (int q = 0; q < NPH; ++q) {
-    const int t = threadIdx.x + q * 256;
+    const int t = threadIdx.x + q * IT;
     ph[q] = (R)0;
     if (t < PN) {
       const int l[3] = {t % PX, (t / PX) % PY, t / (PX * PY)};
@@ -661,36 +662,37 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
   }
 #pragma unroll
   for (int q = 0; q < NPH; ++q) {
-    const int t = threadIdx.x + q * 256;
+    const int t = threadIdx.x + q * IT;
     if (t < PN) lds[t] = ph[q];
   }
   __syncthreads();
   block_scan(cnt, pre, nC, wsum);
   const int total = pre[nC];
-  // 2. force(d, f0 .. f0+IB+1) = sum(phi(m + [-2,-1,1,2]) * weight)
+  // 2. force(d, f0 .. f0+IB+1) = sum(phi(m + [-2,-1,1,2]) * weight), fused
+  //    multiply-adds (one rounding per term; the kick tolerance is 1e-5)
   const FDWeights<R> W;
   R fr[NF][3];
   R f2 = 0;
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
-    const int t = threadIdx.x + q * 256;
+    const int t = threadIdx.x + q * IT;
     if (t < FN) {
       const int lx = t % FX, ly = (t / FX) % FY, lz = t / (FX * FY);
       const int px = lx + 2, py = ly + 2, pz = lz + 2;
       auto P = [&](int x, int y, int z) { return lds[(z * PY + y) * PX + x]; };
       R sx = 0, sy = 0, sz = 0;
-      sx = sx + P(px - 2, py, pz) * W.w0;
-      sx = sx + P(px - 1, py, pz) * W.w1;
-      sx = sx + P(px + 1, py, pz) * W.w2;
-      sx = sx + P(px + 2, py, pz) * W.w3;
-      sy = sy + P(px, py - 2, pz) * W.w0;
-      sy = sy + P(px, py - 1, pz) * W.w1;
-      sy = sy + P(px, py + 1, pz) * W.w2;
-      sy = sy + P(px, py + 2, pz) * W.w3;
-      sz = sz + P(px, py, pz - 2) * W.w0;
-      sz = sz + P(px, py, pz - 1) * W.w1;
-      sz = sz + P(px, py, pz + 1) * W.w2;
-      sz = sz + P(px, py, pz + 2) * W.w3;
+      sx = fmaf(P(px - 2, py, pz), W.w0, sx);
+      sx = fmaf(P(px - 1, py, pz), W.w1, sx);
+      sx = fmaf(P(px + 1, py, pz), W.w2, sx);
+      sx = fmaf(P(px + 2, py, pz), W.w3, sx);
+      sy = fmaf(P(px, py - 2, pz), W.w0, sy);
+      sy = fmaf(P(px, py - 1, pz), W.w1, sy);
+      sy = fmaf(P(px, py + 1, pz), W.w2, sy);
+      sy = fmaf(P(px, py + 2, pz), W.w3, sy);
+      sz = fmaf(P(px, py, pz - 2), W.w0, sz);
+      sz = fmaf(P(px, py, pz - 1), W.w1, sz);
+      sz = fmaf(P(px, py, pz + 1), W.w2, sz);
+      sz = fmaf(P(px, py, pz + 2), W.w3, sz);
       fr[q][0] = sx;
       fr[q][1] = sy;
       fr[q][2] = sz;
@@ -705,7 +707,7 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
-    const int t = threadIdx.x + q * 256;
+    const int t = threadIdx.x + q * IT;
     if (t < FN) {
       lds[t] = fr[q][0];
       lds[FN + t] = fr[q][1];
@@ -717,7 +719,7 @@ __global__ void __launch_bounds__(256) k_interp_cells(PMLevel L, const PMBox* __
   const int sh = XP<IZ>::B - (31 - __clz(w));
   const R rsh = (R)1 / (R)(1u << sh);
   R vm[3] = {0, 0, 0};
-  for (int i = threadIdx.x; i < total; i += 256) {
+  for (int i = threadIdx.x; i < total; i += IT) {
     const int c = find_cell(pre, nC, i);
     const int64_t ip = gof[c] + (i - pre[c]);
     uint32_t uu[3];
@@ -966,7 +968,7 @@ void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* x
     int64_t grid = nbox;
     const int g[3] = {IBX / L.w, IBY / L.w, IBZ / L.w};
     for (int d = 0; d < 3; ++d) grid *= (ncell + g[d] - 1) / g[d];
-    hipLaunchKernelGGL((k_interp_cells<R, IZ>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, (const XT<IZ>*)xp,
+    hipLaunchKernelGGL((k_interp_cells<R, IZ>), dim3((unsigned)grid), dim3(IT), 0, s, L, boxes, (const XT<IZ>*)xp,
                        vp, rhoc, off, phi, a_mid, dt, out);
     CUBE_HIP(hipGetLastError());
     return;

@@ -681,18 +681,18 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
       const int px = lx + 2, py = ly + 2, pz = lz + 2;
       auto P = [&](int x, int y, int z) { return lds[(z * PY + y) * PX + x]; };
       R sx = 0, sy = 0, sz = 0;
-      sx = fmaf(P(px - 2, py, pz), W.w0, sx);
-      sx = fmaf(P(px - 1, py, pz), W.w1, sx);
-      sx = fmaf(P(px + 1, py, pz), W.w2, sx);
-      sx = fmaf(P(px + 2, py, pz), W.w3, sx);
-      sy = fmaf(P(px, py - 2, pz), W.w0, sy);
-      sy = fmaf(P(px, py - 1, pz), W.w1, sy);
-      sy = fmaf(P(px, py + 1, pz), W.w2, sy);
-      sy = fmaf(P(px, py + 2, pz), W.w3, sy);
-      sz = fmaf(P(px, py, pz - 2), W.w0, sz);
-      sz = fmaf(P(px, py, pz - 1), W.w1, sz);
-      sz = fmaf(P(px, py, pz + 1), W.w2, sz);
-      sz = fmaf(P(px, py, pz + 2), W.w3, sz);
+      sx = fma(P(px - 2, py, pz), W.w0, sx);
+      sx = fma(P(px - 1, py, pz), W.w1, sx);
+      sx = fma(P(px + 1, py, pz), W.w2, sx);
+      sx = fma(P(px + 2, py, pz), W.w3, sx);
+      sy = fma(P(px, py - 2, pz), W.w0, sy);
+      sy = fma(P(px, py - 1, pz), W.w1, sy);
+      sy = fma(P(px, py + 1, pz), W.w2, sy);
+      sy = fma(P(px, py + 2, pz), W.w3, sy);
+      sz = fma(P(px, py, pz - 2), W.w0, sz);
+      sz = fma(P(px, py, pz - 1), W.w1, sz);
+      sz = fma(P(px, py, pz + 1), W.w2, sz);
+      sz = fma(P(px, py, pz + 2), W.w3, sz);
       fr[q][0] = sx;
       fr[q][1] = sy;
       fr[q][2] = sz;

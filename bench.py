@@ -234,11 +234,16 @@ def main():
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
     ap.add_argument("--cpu-sample-ng", type=int, default=384, help="0 = skip the CPU baseline")
     ap.add_argument("--pm-adjoint", type=int, default=1, help="0 = skip the PM forward+adjoint timing")
+    ap.add_argument("--pm-adjoint-only", action="store_true",
+                    help="only run the PM forward+adjoint timing and print its JSON object")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
+        return
+    if args.pm_adjoint_only:
+        print(json.dumps(pm_forward_adjoint()), flush=True)
         return
     if args.pm_adjoint:
         import torch  # noqa: F401  (imported first: P3M then lets torch's runtime initialise the device)

@@ -262,59 +262,6 @@ __global__ void k_buffer_rhoc(Geo g, const int32_t* __restrict__ rp, int32_t* __
   if (phys) physcount[((T * g.nt + ijk[2] - 1) * g.nt + ijk[1] - 1) * g.nt + ijk[0] - 1] = v;
 }
 
-// One wavefront per extended cell: the cell's particles are a copy of its
-// periodic source cell's run in the physical layout.  Every lane loads its
-// share of the run (coordinates, velocities, PID) before any store.
-template <int IZ>
-__global__ void k_buffer_copy(Geo g, const int32_t* __restrict__ re, const int64_t* __restrict__ eoff,
-                              const int64_t* __restrict__ poff, const XT<IZ>* __restrict__ xp,
-                              const float* __restrict__ vp, const int64_t* __restrict__ pid, XT<IZ>* __restrict__ xo,
-                              float* __restrict__ vo, int64_t* __restrict__ po) {
-  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (c >= g.ncells_ext) return;
-  const int cnt = re[c];
-  if (cnt == 0) return;
-  const int nte = (int)g.nte, nt = (int)g.nt, nc = (int)g.nc, nnt = (int)g.nnt, ncb = (int)g.ncb;
-  const uint32_t nct = (uint32_t)(nte * nte * nte), cu = (uint32_t)c;
-  const int T = (int)(cu / nct), e = (int)(cu - (uint32_t)T * nct);
-  const int it[3] = {T % nnt, (T / nnt) % nnt, T / (nnt * nnt)};
-  const int ijk[3] = {e % nte + 1 - ncb, (e / nte) % nte + 1 - ncb, e / (nte * nte) + 1 - ncb};
-  int st[3], sc[3];
-  for (int d = 0; d < 3; ++d) {
-    int gc = it[d] * nt + ijk[d] - 1;  // in [-ncb, nc + ncb)
-    gc = gc < 0 ? gc + nc : (gc >= nc ? gc - nc : gc);
-    st[d] = gc / nt;
-    sc[d] = gc - st[d] * nt + 1;
-  }
-  const int Ts = (st[2] * nnt + st[1]) * nnt + st[0];
-  const int64_t src = poff[(((int64_t)Ts * nt + sc[2] - 1) * nt + sc[1] - 1) * nt + sc[0] - 1];
-  const int64_t dst = eoff[c];
-  for (int b = 0; b < cnt; b += 64) {
-    const int n = min(64, cnt - b);
-    XT<IZ> xv[3];
-    float vv[3];
-    int64_t pv = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int q = lane + 64 * k;
-      if (q < 3 * n) {
-        xv[k] = xp[3 * (src + b) + q];
-        vv[k] = vp[3 * (src + b) + q];
-      }
-    }
-    if (pid && lane < n) pv = pid[src + b + lane];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int q = lane + 64 * k;
-      if (q < 3 * n) {
-        xo[3 * (dst + b) + q] = xv[k];
-        vo[3 * (dst + b) + q] = vv[k];
-      }
-    }
-    if (pid && lane < n) po[dst + b + lane] = pv;
-  }
-}
 
 }  // namespace
 
@@ -382,16 +329,72 @@ void buffer_rhoc(const Geo& g, const int32_t* rhoc_phys, int32_t* rhoc_ext, int3
   CUBE_HIP(hipGetLastError());
 }
 
+namespace {
+
+// nn=1 buffer copy by rows: an x-row of an extended tile is three runs of
+// consecutive cells, each a contiguous run of particles in the source
+// layout (cells 1..nt of the tile, the last ncb cells of the periodic left
+// neighbour, the first ncb of the right one) and in the extended layout.
+// One workgroup per (tile, z, y) row copies the three runs with coalesced,
+// independent loads (xp as its 16- or 8-bit elements, vp as floats, pid as
+// 64-bit words), so the copy streams instead of waiting once per cell.
+template <int IZ>
+__global__ void __launch_bounds__(256) k_buffer_rows(Geo g, const int64_t* __restrict__ eoff,
+                                                     const int64_t* __restrict__ poff, const XT<IZ>* __restrict__ xp,
+                                                     const float* __restrict__ vp, const int64_t* __restrict__ pid,
+                                                     XT<IZ>* __restrict__ xo, float* __restrict__ vo,
+                                                     int64_t* __restrict__ po) {
+  const int nte = (int)g.nte, nt = (int)g.nt, nc = (int)g.nc, nnt = (int)g.nnt, ncb = (int)g.ncb;
+  const int64_t r = blockIdx.x;  // (T, ze, ye) with ye fastest
+  const int ye = (int)(r % nte), ze = (int)((r / nte) % nte), T = (int)(r / ((int64_t)nte * nte));
+  const int it[3] = {T % nnt, (T / nnt) % nnt, T / (nnt * nnt)};
+  // source tile and cell of the row's y and z (periodic)
+  int sy, sz, ty, tz;
+  {
+    int gy = it[1] * nt + (ye + 1 - ncb) - 1;
+    gy = gy < 0 ? gy + nc : (gy >= nc ? gy - nc : gy);
+    ty = gy / nt;
+    sy = gy - ty * nt + 1;
+    int gz = it[2] * nt + (ze + 1 - ncb) - 1;
+    gz = gz < 0 ? gz + nc : (gz >= nc ? gz - nc : gz);
+    tz = gz / nt;
+    sz = gz - tz * nt + 1;
+  }
+  const int64_t erow = ((int64_t)T * nte + ze) * nte + ye;  // ext cell index of x_e = 0 is erow * nte
+  for (int seg = 0; seg < 3; ++seg) {
+    // ext x range [xe0, xe1) and its source tile along x
+    const int xe0 = seg == 0 ? 0 : (seg == 1 ? ncb : ncb + nt);
+    const int xe1 = seg == 0 ? ncb : (seg == 1 ? ncb + nt : nte);
+    int gx = it[0] * nt + (xe0 + 1 - ncb) - 1;
+    gx = gx < 0 ? gx + nc : (gx >= nc ? gx - nc : gx);
+    const int tx = gx / nt, sx0 = gx - tx * nt + 1;  // first source cell (1-based)
+    const int Ts = (tz * nnt + ty) * nnt + tx;
+    const int64_t c0 = (((int64_t)Ts * nt + sz - 1) * nt + sy - 1) * nt + sx0 - 1;
+    const int64_t src = poff[c0];
+    const int64_t n = poff[c0 + (xe1 - xe0)] - src;
+    const int64_t dst = eoff[erow * nte + xe0];
+    for (int64_t q = threadIdx.x; q < 3 * n; q += 256) {
+      xo[3 * dst + q] = xp[3 * src + q];
+      vo[3 * dst + q] = vp[3 * src + q];
+    }
+    if (pid)
+      for (int64_t q = threadIdx.x; q < n; q += 256) po[dst + q] = pid[src + q];
+  }
+}
+
+}  // namespace
+
 void buffer_copy(const Geo& g, const int32_t* rhoc_ext, const int64_t* ext_off, const int64_t* phys_off,
                  const void* xp, const float* vp, const int64_t* pid, void* xp_out, float* vp_out, int64_t* pid_out,
                  hipStream_t s) {
-  const int64_t nthr = g.ncells_ext * 64;
+  (void)rhoc_ext;
+  const int64_t rows = g.ntiles * g.nte * g.nte;
   if (g.izipx == 2)
-    hipLaunchKernelGGL(k_buffer_copy<2>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, g, rhoc_ext, ext_off,
-                       phys_off, (const int16_t*)xp, vp, pid, (int16_t*)xp_out, vp_out, pid_out);
+    hipLaunchKernelGGL(k_buffer_rows<2>, dim3((unsigned)rows), dim3(256), 0, s, g, ext_off, phys_off,
+                       (const int16_t*)xp, vp, pid, (int16_t*)xp_out, vp_out, pid_out);
   else
-    hipLaunchKernelGGL(k_buffer_copy<1>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, g, rhoc_ext, ext_off,
-                       phys_off, (const int8_t*)xp, vp, pid, (int8_t*)xp_out, vp_out, pid_out);
+    hipLaunchKernelGGL(k_buffer_rows<1>, dim3((unsigned)rows), dim3(256), 0, s, g, ext_off, phys_off,
+                       (const int8_t*)xp, vp, pid, (int8_t*)xp_out, vp_out, pid_out);
   CUBE_HIP(hipGetLastError());
 }
 

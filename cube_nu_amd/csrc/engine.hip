@@ -98,6 +98,9 @@ struct cube_p3m_ctx {
   DevBuf<PMOut> pmout;
   DevBuf<PPOut> ppout;
   DevBuf<int32_t> ires, napm;
+  DevBuf<int32_t> depcount;
+  DevBuf<int64_t> depstart;
+  DevBuf<unsigned long long> depents;
   DevBuf<int32_t> ppcount;
   DevBuf<int64_t> ppstart;
   DevBuf<float4> ppent;
@@ -247,7 +250,18 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
       if (level == 1)
         pm1_deposit<R>((int)g.izipx, L, xpc(), rhocc(), off.p, stencil.ensure(pm1_stencil_bytes(L)), mesh,
                        in.mass_p_cdm, o, s);
-      else
+      else if (level >= 3 && pm_deposit_binned_applies(L)) {
+        const int64_t nblk = nb * pm_dep_blocks(L);
+        DepBins S;
+        S.count = depcount.ensure((size_t)nblk);
+        S.start = depstart.ensure((size_t)nblk + 1);
+        S.scan_tmp = scan_scratch(nblk);
+        S.scan_tmp_bytes = scan_tmp.n;
+        const int64_t total = pm_dep_bin_count((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
+        S.ents = depents.ensure((size_t)std::max<int64_t>(total, 1));
+        pm_dep_bin_fill((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
+        pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s);
+      } else
         pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, s);
     }
     if (density_host) {

@@ -68,6 +68,29 @@ void green_device(int n, float apm_soft, float apm_range, float gridsize, double
 // Sub-tile classification (kick.f90:110-133): ires(4096) and napm counts.
 void classify_subtiles(const Geo& g, const int32_t* rhoc, int32_t* ires, int32_t* napm, hipStream_t s);
 
+// Binned deposit for sub-tile levels with w <= 8 mesh cells per coarse cell
+// and a power-of-two ratio_cs/gridsize: one pass lists, for every 16^3 mesh
+// block of every box, the particles whose TSC taps reach it (entries carry
+// the box-local cell and the raw coordinates), so the deposit reads each
+// block's contributors only instead of every coarse cell near the block.
+struct DepBins {
+  int32_t* count = nullptr;            // nbox * nblk^3 (counts, then fill cursors)
+  int64_t* start = nullptr;            // nbox * nblk^3 + 1
+  unsigned long long* ents = nullptr;  // entries
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+bool pm_deposit_binned_applies(const PMLevel& L);
+int64_t pm_dep_blocks(const PMLevel& L);  // blocks per box
+// Counts (and scans) the entries; returns their total (host sync).
+int64_t pm_dep_bin_count(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp,
+                         const int32_t* rhoc, const int64_t* cell_off, DepBins& S, hipStream_t s);
+void pm_dep_bin_fill(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp,
+                     const int32_t* rhoc, const int64_t* cell_off, DepBins& S, hipStream_t s);
+template <typename R>
+void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const DepBins& S, R* mesh,
+                      float mass, hipStream_t s);
+
 // Exclusive scan of int32 counts into int64 offsets (n+1 outputs).
 void scan_counts(const int32_t* in, int64_t* out, int64_t n, void* tmp, size_t tmp_bytes, hipStream_t s);
 size_t scan_tmp_bytes(int64_t n);

@@ -290,6 +290,143 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
   }
 }
 
+// Binned deposit, pass 1/2: one wavefront per (box, coarse cell of the box's
+// deposit range).  A coarse cell's floors and taps span at most two 16^3
+// blocks per axis (w <= 8), so each particle belongs to a subset of the 8
+// blocks b0 + {0,1}^3; per block, a ballot counts the members (FILL=0) or
+// places them at start + cursor + rank (FILL=1).  The order inside a list is
+// free: the deposit sums integers, so its result does not depend on it.
+constexpr int kDBD = 16;  // deposit block edge in mesh cells
+
+__host__ __device__ inline int dep_nblk(const PMLevel& L) { return (L.win_hi - L.win_lo + kDBD) / kDBD; }
+
+template <int IZ, int FILL>
+__global__ void __launch_bounds__(256) k_dep_bin(PMLevel L, const PMBox* __restrict__ boxes, int nbox,
+                                                 const XT<IZ>* __restrict__ xp, const int32_t* __restrict__ rhoc,
+                                                 const int64_t* __restrict__ off, int32_t* __restrict__ count,
+                                                 const int64_t* __restrict__ start,
+                                                 unsigned long long* __restrict__ ents) {
+  const int nr = L.nforce / L.w + 2 * L.nloop;
+  const int lane = threadIdx.x & 63;
+  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wv >= (int64_t)nbox * nr * nr * nr) return;
+  const int ib = (int)(wv / (nr * nr * nr)), qc = (int)(wv % (nr * nr * nr));
+  const PMBox B = boxes[ib];
+  const int cl[3] = {qc % nr, (qc / nr) % nr, qc / (nr * nr)};
+  int c[3], lo0[3], b0[3];
+  for (int d = 0; d < 3; ++d) {
+    c[d] = B.nc1[d] - L.nloop + cl[d];
+    lo0[d] = L.w * (c[d] - 1) - B.ushift[d];  // the cell's lowest floor
+    b0[d] = (max(lo0[d], L.win_lo) - L.win_lo) / kDBD;
+  }
+  const int64_t cell = rc_index(c[0], c[1], c[2], B.T, L.nte, L.ncb);
+  const int cnt = rhoc[cell];
+  const int64_t base = off[cell];
+  const int nb = dep_nblk(L);
+  const int sh = XP<IZ>::B - (31 - __clz(L.w));
+  const int64_t bbase = (int64_t)ib * nb * nb * nb;
+  const unsigned long long ckey = (unsigned long long)qc << 48;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int l0 = 0; l0 < cnt; l0 += 64) {
+    const int l = l0 + lane;
+    uint32_t u[3] = {0, 0, 0};
+    unsigned mem = 0;  // bit ox + 2 oy + 4 oz: member of block b0 + (ox, oy, oz)
+    if (l < cnt) {
+      int rlo[3], rhi[3];
+      bool any = true;
+      for (int d = 0; d < 3; ++d) {
+        u[d] = XP<IZ>::raw(xp, base + l, d);
+        const int f = lo0[d] + (int)(u[d] >> sh);
+        const int tl = max(f, L.win_lo), th = min(f + 2, L.win_hi);
+        any = any && tl <= th;
+        rlo[d] = (tl - L.win_lo) / kDBD - b0[d];
+        rhi[d] = (th - L.win_lo) / kDBD - b0[d];
+      }
+      if (any)
+        for (int j = 0; j < 8; ++j) {
+          const int o[3] = {j & 1, (j >> 1) & 1, j >> 2};
+          bool in = true;
+          for (int d = 0; d < 3; ++d) in = in && o[d] >= rlo[d] && o[d] <= rhi[d];
+          if (in) mem |= 1u << j;
+        }
+    }
+    const unsigned long long e = ckey | ((unsigned long long)u[2] << 32) | ((unsigned long long)u[1] << 16) | u[0];
+    for (int j = 0; j < 8; ++j) {
+      const unsigned long long m = __ballot((mem >> j) & 1u);
+      if (m == 0) continue;
+      const int64_t blk = bbase + ((int64_t)(b0[2] + (j >> 2)) * nb + (b0[1] + ((j >> 1) & 1))) * nb + b0[0] + (j & 1);
+      const int n = (int)__popcll(m);
+      int pos0 = 0;
+      if (lane == 0) pos0 = atomicAdd(&count[blk], n);
+      if (FILL) {
+        pos0 = __shfl(pos0, 0, 64);
+        if ((mem >> j) & 1u) ents[start[blk] + pos0 + (int)__popcll(m & lt)] = e;
+      }
+    }
+  }
+}
+
+// Binned deposit, pass 2/2: one workgroup per 16^3 block of one box, the
+// same regions as k_deposit's mesh-aligned blocks; each thread deposits one
+// listed particle at a time (taps outside the block skipped), then the block
+// writes its mesh cells.
+template <typename R, int IZ>
+__global__ void __launch_bounds__(256) k_deposit_lists(PMLevel L, const PMBox* __restrict__ boxes,
+                                                       const int64_t* __restrict__ start,
+                                                       const unsigned long long* __restrict__ ents,
+                                                       R* __restrict__ mesh, float mass) {
+  using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
+  constexpr int BD = kDBD;
+  __shared__ Acc acc[BD * BD * BD];
+  const int nb = dep_nblk(L), nr = L.nforce / L.w + 2 * L.nloop;
+  const int64_t blk = blockIdx.x;
+  int64_t b = blk;
+  const int bx = (int)(b % nb);
+  b /= nb;
+  const int by = (int)(b % nb);
+  b /= nb;
+  const int bz = (int)(b % nb);
+  const int ibox = (int)(b / nb);
+  const PMBox B = boxes[ibox];
+  const int bidx[3] = {bx, by, bz};
+  int mlo[3], mhi[3];
+  for (int d = 0; d < 3; ++d) {
+    mlo[d] = L.win_lo + bidx[d] * BD;
+    mhi[d] = min(mlo[d] + BD - 1, L.win_hi);
+  }
+  for (int t = threadIdx.x; t < BD * BD * BD; t += blockDim.x) acc[t] = 0;
+  __syncthreads();
+  const int sh = XP<IZ>::B - (31 - __clz(L.w));
+  const R rsh = (R)1 / (R)(1u << sh);
+  const int64_t e0 = start[blk], e1 = start[blk + 1];
+  for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+    const unsigned long long e = ents[i];
+    const int qc = (int)(e >> 48);
+    const int cl[3] = {qc % nr, (qc / nr) % nr, qc / (nr * nr)};
+    int f[3];
+    R wt[3][3];
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t u = (uint32_t)(e >> (16 * d)) & 0xFFFFu;
+      const int c = B.nc1[d] - L.nloop + cl[d];
+      f[d] = L.w * (c - 1) - B.ushift[d] + (int)(u >> sh);
+      tsc_weights_frac<R>(((R)(u & ((1u << sh) - 1u)) + (R)0.5) * rsh, wt[d]);
+    }
+    deposit_particle<R, Acc, BD>(acc, f, wt, mlo, mhi);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < BD * BD * BD; t += blockDim.x) {
+    const int lx = t % BD, ly = (t / BD) % BD, lz = t / (BD * BD);
+    const int m[3] = {mlo[0] + lx, mlo[1] + ly, mlo[2] + lz};
+    if (m[0] > mhi[0] || m[1] > mhi[1] || m[2] > mhi[2]) continue;
+    R v;
+    if constexpr (std::is_same<R, float>::value) v = (float)((double)acc[t] * (1.0 / (double)kFixF));
+    else v = acc[t];
+    int a[3];
+    for (int d = 0; d < 3; ++d) a[d] = m[d] + B.org[d] - L.m0;
+    mesh[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]] = v * (R)mass;
+  }
+}
+
 // Coarse-level deposit (PM1), step 1.  Every particle of coarse cell c has
 // floor(xpos) = c-1 and deposits into the same 27 mesh cells c-1..c+1
 // (particle_mesh.f90:56-58), so one wavefront sums the 27 tap weights of a
@@ -984,6 +1121,70 @@ void pm_deposit(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const
   if (izipx == 2) launch_deposit<R, 2>(L, boxes, nbox, xp, rhoc, off, mesh, mass, s);
   else launch_deposit<R, 1>(L, boxes, nbox, xp, rhoc, off, mesh, mass, s);
 }
+
+bool pm_deposit_binned_applies(const PMLevel& L) {
+  // w = 8 (two blocks per axis per coarse cell), power-of-two xmul, and the
+  // window of the deposited cells' taps; the box-local cell index fits 16 bits
+  const int nr = L.nforce / L.w + 2 * L.nloop;
+  return !L.periodic && !L.xdiv && L.w == 8 && L.nforce % L.w == 0 && L.win_lo == 1 - L.nloop * L.w &&
+         L.win_hi == L.nforce + L.nloop * L.w && (int64_t)nr * nr * nr < 65536;
+}
+
+int64_t pm_dep_blocks(const PMLevel& L) {
+  const int64_t nb = dep_nblk(L);
+  return nb * nb * nb;
+}
+
+namespace {
+template <int FILL>
+void launch_dep_bin(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
+                    const int64_t* off, DepBins& S, hipStream_t s) {
+  const int nr = L.nforce / L.w + 2 * L.nloop;
+  const int64_t nthr = (int64_t)nbox * nr * nr * nr * 64;
+  if (izipx == 2)
+    hipLaunchKernelGGL((k_dep_bin<2, FILL>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
+                       (const int16_t*)xp, rhoc, off, S.count, S.start, S.ents);
+  else
+    hipLaunchKernelGGL((k_dep_bin<1, FILL>), dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, L, boxes, nbox,
+                       (const int8_t*)xp, rhoc, off, S.count, S.start, S.ents);
+  CUBE_HIP(hipGetLastError());
+}
+}  // namespace
+
+int64_t pm_dep_bin_count(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp,
+                         const int32_t* rhoc, const int64_t* off, DepBins& S, hipStream_t s) {
+  const int64_t nblk = (int64_t)nbox * pm_dep_blocks(L);
+  zero_async(S.count, sizeof(int32_t) * nblk, s);
+  launch_dep_bin<0>(izipx, L, boxes, nbox, xp, rhoc, off, S, s);
+  scan_counts(S.count, S.start, nblk, S.scan_tmp, S.scan_tmp_bytes, s);
+  int64_t total = 0;
+  CUBE_HIP(hipMemcpyAsync(&total, S.start + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  return total;
+}
+
+void pm_dep_bin_fill(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, const int32_t* rhoc,
+                     const int64_t* off, DepBins& S, hipStream_t s) {
+  zero_async(S.count, sizeof(int32_t) * nbox * pm_dep_blocks(L), s);
+  launch_dep_bin<1>(izipx, L, boxes, nbox, xp, rhoc, off, S, s);
+}
+
+template <typename R>
+void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const DepBins& S, R* mesh,
+                      float mass, hipStream_t s) {
+  const int64_t grid = (int64_t)nbox * pm_dep_blocks(L);
+  if (izipx == 2)
+    hipLaunchKernelGGL((k_deposit_lists<R, 2>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.ents,
+                       mesh, mass);
+  else
+    hipLaunchKernelGGL((k_deposit_lists<R, 1>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.ents,
+                       mesh, mass);
+  CUBE_HIP(hipGetLastError());
+}
+template void pm_deposit_lists<float>(int, const PMLevel&, const PMBox*, int, const DepBins&, float*, float,
+                                      hipStream_t);
+template void pm_deposit_lists<double>(int, const PMLevel&, const PMBox*, int, const DepBins&, double*, float,
+                                       hipStream_t);
 
 void pm_level_scale(PMLevel& L) {
   int e = 0;

@@ -351,16 +351,26 @@ __global__ void __launch_bounds__(256) k_dep_bin(PMLevel L, const PMBox* __restr
         }
     }
     const unsigned long long e = ckey | ((unsigned long long)u[2] << 32) | ((unsigned long long)u[1] << 16) | u[0];
+    // lane j < 8 owns block b0 + o(j): its member mask, one atomic for the
+    // wavefront's members and (FILL) the list position; the eight atomics
+    // and start loads are independent, so they overlap
+    unsigned long long mine = 0;
     for (int j = 0; j < 8; ++j) {
       const unsigned long long m = __ballot((mem >> j) & 1u);
-      if (m == 0) continue;
-      const int64_t blk = bbase + ((int64_t)(b0[2] + (j >> 2)) * nb + (b0[1] + ((j >> 1) & 1))) * nb + b0[0] + (j & 1);
-      const int n = (int)__popcll(m);
-      int pos0 = 0;
-      if (lane == 0) pos0 = atomicAdd(&count[blk], n);
-      if (FILL) {
-        pos0 = __shfl(pos0, 0, 64);
-        if ((mem >> j) & 1u) ents[start[blk] + pos0 + (int)__popcll(m & lt)] = e;
+      if (lane == j) mine = m;
+    }
+    int64_t where = 0;
+    if (lane < 8 && mine != 0) {
+      const int64_t blk =
+          bbase + ((int64_t)(b0[2] + (lane >> 2)) * nb + (b0[1] + ((lane >> 1) & 1))) * nb + b0[0] + (lane & 1);
+      const int pos = atomicAdd(&count[blk], (int)__popcll(mine));
+      if (FILL) where = start[blk] + pos;
+    }
+    if (FILL) {
+      for (int j = 0; j < 8; ++j) {
+        const int64_t wj = __shfl(where, j, 64);
+        const unsigned long long mj = __shfl(mine, j, 64);
+        if ((mem >> j) & 1u) ents[wj + (int64_t)__popcll(mj & lt)] = e;
       }
     }
   }

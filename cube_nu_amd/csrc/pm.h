@@ -74,15 +74,16 @@ void classify_subtiles(const Geo& g, const int32_t* rhoc, int32_t* ires, int32_t
 // the box-local cell and the raw coordinates), so the deposit reads each
 // block's contributors only instead of every coarse cell near the block.
 struct DepBins {
-  int32_t* count = nullptr;            // nbox * nblk^3 (counts, then fill cursors)
-  int64_t* start = nullptr;            // nbox * nblk^3 + 1
+  int32_t* count = nullptr;            // nbox * nblk^3 (capacities, then entries used)
+  int64_t* start = nullptr;            // nbox * nblk^3 + 1 (scan of the capacities)
   unsigned long long* ents = nullptr;  // entries
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
 };
 bool pm_deposit_binned_applies(const PMLevel& L);
 int64_t pm_dep_blocks(const PMLevel& L);  // blocks per box
-// Counts (and scans) the entries; returns their total (host sync).
+// List capacities (the particles of the coarse cells that can reach each
+// block) and their scan; returns the total capacity (host sync).
 int64_t pm_dep_bin_count(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp,
                          const int32_t* rhoc, const int64_t* cell_off, DepBins& S, hipStream_t s);
 void pm_dep_bin_fill(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp,

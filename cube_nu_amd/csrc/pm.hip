@@ -359,21 +359,49 @@ __global__ void __launch_bounds__(256) k_dep_bin(PMLevel L, const PMBox* __restr
       const unsigned long long m = __ballot((mem >> j) & 1u);
       if (lane == j) mine = m;
     }
-    int64_t where = 0;
+    int64_t where = 0, lim = 0;
     if (lane < 8 && mine != 0) {
       const int64_t blk =
           bbase + ((int64_t)(b0[2] + (lane >> 2)) * nb + (b0[1] + ((lane >> 1) & 1))) * nb + b0[0] + (lane & 1);
       const int pos = atomicAdd(&count[blk], (int)__popcll(mine));
-      if (FILL) where = start[blk] + pos;
+      if (FILL) {
+        where = start[blk] + pos;
+        lim = start[blk + 1];  // end of the block's capacity
+      }
     }
     if (FILL) {
       for (int j = 0; j < 8; ++j) {
-        const int64_t wj = __shfl(where, j, 64);
+        const int64_t wj = __shfl(where, j, 64), lj = __shfl(lim, j, 64);
         const unsigned long long mj = __shfl(mine, j, 64);
-        if ((mem >> j) & 1u) ents[wj + (int64_t)__popcll(mj & lt)] = e;
+        const int64_t at = wj + (int64_t)__popcll(mj & lt);
+        if (((mem >> j) & 1u) && at < lj) ents[at] = e;
       }
     }
   }
+}
+
+// List capacity of each block: the particles of every coarse cell whose taps
+// can reach it (the cells a gather-form block would read), summed from the
+// counts, so that no pass over the particles is needed to size the lists.
+__global__ void k_dep_cap(PMLevel L, const PMBox* __restrict__ boxes, int nbox, const int32_t* __restrict__ rhoc,
+                          int32_t* __restrict__ cap) {
+  const int nb = dep_nblk(L);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nbox * nb * nb * nb) return;
+  int64_t b = t;
+  const int bi[3] = {(int)(b % nb), (int)((b / nb) % nb), (int)((b / ((int64_t)nb * nb)) % nb)};
+  const PMBox B = boxes[b / ((int64_t)nb * nb * nb)];
+  int clo[3], chi[3];
+  for (int d = 0; d < 3; ++d) {
+    const int mlo = L.win_lo + bi[d] * kDBD, mhi = min(mlo + kDBD - 1, L.win_hi);
+    clo[d] = max(-floor_div(-(mlo - 1 + B.ushift[d]), L.w), B.nc1[d] - L.nloop);
+    chi[d] = min(floor_div(mhi + B.ushift[d], L.w) + 1, B.nc2[d] + L.nloop);
+  }
+  int sum = 0;
+  for (int k = clo[2]; k <= chi[2]; ++k)
+    for (int j = clo[1]; j <= chi[1]; ++j)
+      for (int i = clo[0]; i <= chi[0]; ++i) sum += rhoc[rc_index(i, j, k, B.T, L.nte, L.ncb)];
+  cap[t] = sum;
 }
 
 // Binned deposit, pass 2/2: one workgroup per 16^3 block of one box, the
@@ -383,6 +411,7 @@ __global__ void __launch_bounds__(256) k_dep_bin(PMLevel L, const PMBox* __restr
 template <typename R, int IZ>
 __global__ void __launch_bounds__(256) k_deposit_lists(PMLevel L, const PMBox* __restrict__ boxes,
                                                        const int64_t* __restrict__ start,
+                                                       const int32_t* __restrict__ used,
                                                        const unsigned long long* __restrict__ ents,
                                                        R* __restrict__ mesh, float mass) {
   using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
@@ -408,7 +437,7 @@ __global__ void __launch_bounds__(256) k_deposit_lists(PMLevel L, const PMBox* _
   __syncthreads();
   const int sh = XP<IZ>::B - (31 - __clz(L.w));
   const R rsh = (R)1 / (R)(1u << sh);
-  const int64_t e0 = start[blk], e1 = start[blk + 1];
+  const int64_t e0 = start[blk], e1 = e0 + used[blk];
   for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
     const unsigned long long e = ents[i];
     const int qc = (int)(e >> 48);
@@ -1163,9 +1192,12 @@ void launch_dep_bin(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, c
 
 int64_t pm_dep_bin_count(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp,
                          const int32_t* rhoc, const int64_t* off, DepBins& S, hipStream_t s) {
+  (void)izipx;
+  (void)xp;
+  (void)off;
   const int64_t nblk = (int64_t)nbox * pm_dep_blocks(L);
-  zero_async(S.count, sizeof(int32_t) * nblk, s);
-  launch_dep_bin<0>(izipx, L, boxes, nbox, xp, rhoc, off, S, s);
+  hipLaunchKernelGGL(k_dep_cap, dim3((unsigned)ceil_div(nblk, 256)), dim3(256), 0, s, L, boxes, nbox, rhoc, S.count);
+  CUBE_HIP(hipGetLastError());
   scan_counts(S.count, S.start, nblk, S.scan_tmp, S.scan_tmp_bytes, s);
   int64_t total = 0;
   CUBE_HIP(hipMemcpyAsync(&total, S.start + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -1184,11 +1216,11 @@ void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes, int nbox,
                       float mass, hipStream_t s) {
   const int64_t grid = (int64_t)nbox * pm_dep_blocks(L);
   if (izipx == 2)
-    hipLaunchKernelGGL((k_deposit_lists<R, 2>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.ents,
-                       mesh, mass);
+    hipLaunchKernelGGL((k_deposit_lists<R, 2>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.count,
+                       S.ents, mesh, mass);
   else
-    hipLaunchKernelGGL((k_deposit_lists<R, 1>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.ents,
-                       mesh, mass);
+    hipLaunchKernelGGL((k_deposit_lists<R, 1>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.count,
+                       S.ents, mesh, mass);
   CUBE_HIP(hipGetLastError());
 }
 template void pm_deposit_lists<float>(int, const PMLevel&, const PMBox*, int, const DepBins&, float*, float,

@@ -124,7 +124,8 @@ __global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restri
 }
 
 constexpr int PB = 4;       // physical PP cells per brick per dim
-constexpr int kUnroll = kPPPad;  // candidate loads in flight per lane
+constexpr int kUnroll = 6;  // candidate loads in flight per lane (measured best of 4, 6, 8)
+static_assert(kUnroll <= kPPPad, "entries are padded for kUnroll reads past a run");
 
 
 // FP32 pair law F(r, app) - F(r, r_pp) for 0 < r < r_pp.  F(r, r_pp) is one

@@ -325,7 +325,9 @@ __global__ void __launch_bounds__(256) k_dep_bin(PMLevel L, const PMBox* __restr
   const int nb = dep_nblk(L);
   const int sh = XP<IZ>::B - (31 - __clz(L.w));
   const int64_t bbase = (int64_t)ib * nb * nb * nb;
-  const unsigned long long ckey = (unsigned long long)qc << 48;
+  // box-local cell as three 5-bit fields (nr <= 32), so the deposit decodes
+  // it without divisions
+  const unsigned long long ckey = (unsigned long long)(cl[0] | (cl[1] << 5) | (cl[2] << 10)) << 48;
   const unsigned long long lt = (1ull << lane) - 1ull;
   for (int l0 = 0; l0 < cnt; l0 += 64) {
     const int l = l0 + lane;
@@ -417,7 +419,7 @@ __global__ void __launch_bounds__(256) k_deposit_lists(PMLevel L, const PMBox* _
   using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
   constexpr int BD = kDBD;
   __shared__ Acc acc[BD * BD * BD];
-  const int nb = dep_nblk(L), nr = L.nforce / L.w + 2 * L.nloop;
+  const int nb = dep_nblk(L);
   const int64_t blk = blockIdx.x;
   int64_t b = blk;
   const int bx = (int)(b % nb);
@@ -441,7 +443,7 @@ __global__ void __launch_bounds__(256) k_deposit_lists(PMLevel L, const PMBox* _
   for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
     const unsigned long long e = ents[i];
     const int qc = (int)(e >> 48);
-    const int cl[3] = {qc % nr, (qc / nr) % nr, qc / (nr * nr)};
+    const int cl[3] = {qc & 31, (qc >> 5) & 31, (qc >> 10) & 31};
     int f[3];
     R wt[3][3];
     for (int d = 0; d < 3; ++d) {
@@ -1163,10 +1165,10 @@ void pm_deposit(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const
 
 bool pm_deposit_binned_applies(const PMLevel& L) {
   // w = 8 (two blocks per axis per coarse cell), power-of-two xmul, and the
-  // window of the deposited cells' taps; the box-local cell index fits 16 bits
+  // window of the deposited cells' taps; box-local cell coordinates < 32
   const int nr = L.nforce / L.w + 2 * L.nloop;
   return !L.periodic && !L.xdiv && L.w == 8 && L.nforce % L.w == 0 && L.win_lo == 1 - L.nloop * L.w &&
-         L.win_hi == L.nforce + L.nloop * L.w && (int64_t)nr * nr * nr < 65536;
+         L.win_hi == L.nforce + L.nloop * L.w && nr <= 32;
 }
 
 int64_t pm_dep_blocks(const PMLevel& L) {

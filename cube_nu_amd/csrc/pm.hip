@@ -207,8 +207,8 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
     wave_sync_lds();
     if (lane < nq) {
       const unsigned long long e = q[lane];
-      const int c = (int)(e >> 48);
-      const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+      const int c = (int)(e >> 48);  // staged cell as 5-bit (x, y, z) offsets from clo
+      const int ijk[3] = {clo[0] + (c & 31), clo[1] + ((c >> 5) & 31), clo[2] + ((c >> 10) & 31)};
       int f[3];
       R wt[3][3];
       for (int d = 0; d < 3; ++d) {
@@ -245,10 +245,11 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
       }
 #pragma unroll
       for (int k = 0; k < G; ++k) {
-        const int c = c0 + k * nw;
+        const int c = c0 + k * nw;  // wave-uniform
+        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
         bool hit = valid[k];
         if (hit) {
-          const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+          const int ijk[3] = {clo[0] + cx, clo[1] + cy, clo[2] + cz};
           for (int d = 0; d < 3; ++d) {
             int f;
             if (sh >= 0) f = L.w * (ijk[d] - 1) - B.ushift[d] + (int)(u[k][d] >> sh);
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
         const unsigned long long m = __ballot(hit);
         if (hit) {
           const int pos = qn + (int)__popcll(m & ((1ull << lane) - 1ull));
-          q[pos] = ((unsigned long long)c << 48) | ((unsigned long long)u[k][2] << 32) |
+          q[pos] = ((unsigned long long)(cx | (cy << 5) | (cz << 10)) << 48) | ((unsigned long long)u[k][2] << 32) |
                    ((unsigned long long)u[k][1] << 16) | (unsigned long long)u[k][0];
         }
         qn += (int)__popcll(m);

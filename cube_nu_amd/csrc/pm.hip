@@ -790,7 +790,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   constexpr int MAXC = 16;
   __shared__ R lds[LDSN];
   __shared__ R red[4];
-  __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4];
+  __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4], cxyz[MAXC];
   __shared__ int64_t gof[MAXC];
   const int w = L.w;
   const int g[3] = {IBX / w, IBY / w, IBZ / w};  // coarse cells per block per axis
@@ -834,10 +834,11 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   const int ncx = chi[0] - clo[0] + 1, ncy = chi[1] - clo[1] + 1, ncz = chi[2] - clo[2] + 1;
   const int nC = ncx * ncy * ncz;
   for (int t = threadIdx.x; t < nC; t += blockDim.x) {
-    const int64_t cell = rc_index(clo[0] + t % ncx, clo[1] + (t / ncx) % ncy, clo[2] + t / (ncx * ncy), B.T, L.nte,
-                                  L.ncb);
+    const int cx = t % ncx, cy = (t / ncx) % ncy, cz = t / (ncx * ncy);
+    const int64_t cell = rc_index(clo[0] + cx, clo[1] + cy, clo[2] + cz, B.T, L.nte, L.ncb);
     cnt[t] = rhoc[cell];
     gof[t] = off[cell];
+    cxyz[t] = cx | (cy << 5) | (cz << 10);
   }
 #pragma unroll
   for (int q = 0; q < NPH; ++q) {
@@ -907,7 +908,8 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
       uu[d] = XP<IZ>::raw(xp, ip, d);
       v0[d] = vp[3 * ip + d];
     }
-    const int ijk[3] = {clo[0] + c % ncx, clo[1] + (c / ncx) % ncy, clo[2] + c / (ncx * ncy)};
+    const int cc = cxyz[c];
+    const int ijk[3] = {clo[0] + (cc & 31), clo[1] + ((cc >> 5) & 31), clo[2] + ((cc >> 10) & 31)};
     R wt[3][3];
     int f[3];
     bool ok = true;

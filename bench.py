@@ -222,6 +222,97 @@ def pm_forward_adjoint(reps=2):
             "fd_directional_rel_err": fd_err}
 
 
+def _rel(a, b, chunk=1 << 24):
+    """sqrt(sum (a-b)^2 / sum b^2) in FP64, in chunks."""
+    import numpy as np
+
+    a, b = a.reshape(-1), b.reshape(-1)
+    num = den = 0.0
+    for i in range(0, a.size, chunk):
+        x, y = a[i:i + chunk].astype(np.float64), b[i:i + chunk].astype(np.float64)
+        num += float(np.dot(x - y, x - y))
+        den += float(np.dot(y, y))
+    return (num / den) ** 0.5 if den > 0 else 0.0
+
+
+def _positions(geo, xp, rhoc):
+    """Global coarse-cell coordinates (N, 3) of a physical layout: cell
+    (i, j, k) of tile (tx, ty, tz) holds rhoc particles in storage order,
+    x = tx nt + (i - 1) + (u + 1/2) 2^-B with u the unsigned coordinate."""
+    import numpy as np
+
+    nte, nt, nnt, ncb = geo.nte, geo.nt, geo.nnt, geo.ncb
+    counts = rhoc.reshape(-1).astype(np.int64)
+    cell = np.repeat(np.arange(counts.size, dtype=np.int64), counts)
+    B = 8 * geo.izipx
+    u = xp.astype(np.int64) & ((1 << B) - 1)
+    out = np.empty((cell.size, 3), np.float64)
+    for d in range(3):
+        c = (cell // nte ** d) % nte
+        t = (cell // (nte ** 3 * nnt ** d)) % nnt
+        out[:, d] = t * nt + (c - ncb) + (u[:, d] + 0.5) / float(1 << B)
+    return out
+
+
+def precision_study(ng, z, box):
+    """Error of the FP32 form against the FP64 form (the same kernels in
+    double) per field, on one seeded IC: the PM1 density, the velocity change
+    of each kick phase (PM1, PM2, PM3, PP) and of the full kick (each from
+    zero velocities, so the difference is the force times the kick factor),
+    and after a full kick + drift the positions and velocities matched by
+    particle ID."""
+    import numpy as np
+    from cube_nu_amd import P3M
+
+    a = 1.0 / (1.0 + z)
+    p = P3M(ng, izipx=2, precision=32, box=box)
+    p.make_ic(seed=20260101, z=z, nic=2)
+    k0 = p.kick_resident(a_mid=a, dt=0.0)
+    dt = min(k0["dt_pm1"], k0["dt_pm2"], k0["dt_pm3"], k0["dt_pp"], 1.0)
+    xp, vp, pid, rhoc = p.download()
+    p.close()
+    geo = p.geo
+    phases = (("pm1", 1), ("pm2", 2), ("pm3", 4), ("pp", 8), ("kick", 0))
+    res, vz = {32: {}, 64: {}}, None
+    for prec in (32, 64):
+        t0 = time.time()
+        q = P3M(ng, izipx=2, precision=prec, box=box)
+        r = res[prec]
+        r["rho1"], _ = q.particle_mesh(xp, np.zeros_like(vp), rhoc, a_mid=a, dt=dt)
+        for name, mask in phases:
+            dv = np.zeros_like(vp)
+            k = q.kick(xp, dv, rhoc, a_mid=a, dt=dt, phase_mask=mask)
+            r[name] = dv
+            if name == "kick":
+                r["napm"] = k["napm"]
+        q.upload(xp, vp, pid, rhoc)
+        k = q.kick_resident(a_mid=a, dt=dt)
+        q.buffer_v_resident()
+        vz = k["vz_max"] if vz is None else vz  # the same colour count in both drifts
+        q.drift_resident(dt_old=dt, dt=dt, vz_max=vz)
+        x2, v2, p2, rc2 = q.download(cap=xp.shape[0])
+        q.close()
+        inv = np.empty(p2.size, np.int64)
+        inv[p2 - 1] = np.arange(p2.size)
+        r["x"] = _positions(geo, x2, rc2)[inv]
+        r["v"] = v2[inv]
+        r["xp"], r["rhoc"] = x2[inv], rc2
+        log(f"[bench] precision study fp{prec}: {time.time() - t0:.1f}s")
+    A, B = res[32], res[64]
+    out = {"config": f"{ng}^3, box {box:g} Mpc/h, z={z:g}, dt={dt:.4g}, napm={A['napm']}",
+           "rms_rel": {"pm1_density": _rel(A["rho1"], B["rho1"])}}
+    for name, _ in phases:
+        out["rms_rel"][f"dv_{name}"] = _rel(A[name], B[name])
+    out["rms_rel"]["v_after_step"] = _rel(A["v"], B["v"])
+    d = A["x"] - B["x"]
+    d -= geo.nc * np.round(d / geo.nc)
+    out["positions"] = {"identical_frac": float(np.mean(np.all(A["xp"] == B["xp"], axis=1))),
+                        "rhoc_identical": bool(np.array_equal(A["rhoc"], B["rhoc"])),
+                        "rms_dx_coarse_cells": float(np.sqrt(np.mean(d * d))),
+                        "max_dx_coarse_cells": float(np.abs(d).max())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +325,11 @@ def main():
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
     ap.add_argument("--cpu-sample-ng", type=int, default=384, help="0 = skip the CPU baseline")
     ap.add_argument("--pm-adjoint", type=int, default=1, help="0 = skip the PM forward+adjoint timing")
+    ap.add_argument("--precision-study", type=int, default=1,
+                    help="0 = skip the per-field FP32-vs-FP64 comparison")
+    ap.add_argument("--precision-z", type=float, default=None, help="redshift of that comparison (default --z)")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only run the FP32-vs-FP64 comparison and print its JSON object")
     ap.add_argument("--pm-adjoint-only", action="store_true",
                     help="only run the PM forward+adjoint timing and print its JSON object")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -244,6 +340,10 @@ def main():
         return
     if args.pm_adjoint_only:
         print(json.dumps(pm_forward_adjoint()), flush=True)
+        return
+    zp = args.z if args.precision_z is None else args.precision_z
+    if args.precision_only:
+        print(json.dumps(precision_study(args.ng, zp, args.box)), flush=True)
         return
     if args.pm_adjoint:
         import torch  # noqa: F401  (imported first: P3M then lets torch's runtime initialise the device)
@@ -271,6 +371,12 @@ def main():
             pm_adj = pm_forward_adjoint()
         except Exception as e:  # reported, not hidden
             pm_adj = {"error": str(e)}
+    prec = None
+    if args.precision_study:
+        try:
+            prec = precision_study(args.ng, zp, args.box)
+        except Exception as e:  # reported, not hidden
+            prec = {"error": str(e)}
     cpu = None
     if args.cpu_sample_ng > 0:
         try:
@@ -286,7 +392,7 @@ def main():
                                "(drift + buffer + PM1/PM2/PM3/PP kick), int16 coordinates",
                    "ng": args.ng, "izipx": 2, "dt": r32["dt"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64_form": fp64, "fp64_plain_per_tile": fp64_plain,
-        "phase_ms_per_step": phases, "pm_forward_adjoint": pm_adj,
+        "phase_ms_per_step": phases, "pm_forward_adjoint": pm_adj, "fp32_vs_fp64": prec,
     }
     print(json.dumps(line), flush=True)
 

@@ -49,5 +49,9 @@ void buffer_rhoc(const Geo& g, const int32_t* rhoc_phys, int32_t* rhoc_ext, int3
 void buffer_copy(const Geo& g, const int32_t* rhoc_ext, const int64_t* ext_off, const int64_t* phys_off,
                  const void* xp, const float* vp, const int64_t* pid, void* xp_out, float* vp_out, int64_t* pid_out,
                  hipStream_t s);
+// buffer_v: in the extended layout (offsets ext_off, the scan of the extended
+// counts), buffer velocities (and IDs, when pid is given) from the physical
+// cells they are images of.
+void buffer_v(const Geo& g, const int64_t* ext_off, float* vp, int64_t* pid, hipStream_t s);
 
 }  // namespace cube

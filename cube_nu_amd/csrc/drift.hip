@@ -382,7 +382,62 @@ __global__ void __launch_bounds__(256) k_buffer_rows(Geo g, const int64_t* __res
   }
 }
 
+// buffer_v at nn=1 (buffer_particle.f90:11-27, buffer_xvp(.false., .true.)):
+// in the extended layout, the velocities (and IDs) of every buffer cell are
+// overwritten by those of the physical cell it is the periodic image of.  The
+// same three runs per (tile, z, y) row as k_buffer_rows, read from the
+// physical cells of the extended array itself: sources are physical cells and
+// destinations buffer cells, so no run is both.  A run copies the smaller of
+// its source and destination counts (equal in a layout made by the buffer
+// step), so every access stays inside both runs.
+template <bool PID>
+__global__ void __launch_bounds__(256) k_buffer_v_rows(Geo g, const int64_t* __restrict__ eoff, float* vp,
+                                                       int64_t* pid) {
+  const int nte = (int)g.nte, nt = (int)g.nt, nc = (int)g.nc, nnt = (int)g.nnt, ncb = (int)g.ncb;
+  const int64_t r = blockIdx.x;  // (T, ze, ye) with ye fastest
+  const int ye = (int)(r % nte), ze = (int)((r / nte) % nte), T = (int)(r / ((int64_t)nte * nte));
+  const int it[3] = {T % nnt, (T / nnt) % nnt, T / (nnt * nnt)};
+  const bool yz_phys = ye >= ncb && ye < ncb + nt && ze >= ncb && ze < ncb + nt;
+  int sy, sz, ty, tz;
+  {
+    int gy = it[1] * nt + (ye + 1 - ncb) - 1;
+    gy = gy < 0 ? gy + nc : (gy >= nc ? gy - nc : gy);
+    ty = gy / nt;
+    sy = gy - ty * nt + 1;
+    int gz = it[2] * nt + (ze + 1 - ncb) - 1;
+    gz = gz < 0 ? gz + nc : (gz >= nc ? gz - nc : gz);
+    tz = gz / nt;
+    sz = gz - tz * nt + 1;
+  }
+  const int64_t erow = ((int64_t)T * nte + ze) * nte + ye;
+  for (int seg = 0; seg < 3; ++seg) {
+    if (seg == 1 && yz_phys) continue;  // the row's physical cells are their own source
+    const int xe0 = seg == 0 ? 0 : (seg == 1 ? ncb : ncb + nt);
+    const int xe1 = seg == 0 ? ncb : (seg == 1 ? ncb + nt : nte);
+    int gx = it[0] * nt + (xe0 + 1 - ncb) - 1;
+    gx = gx < 0 ? gx + nc : (gx >= nc ? gx - nc : gx);
+    const int tx = gx / nt, sx0 = gx - tx * nt + 1;
+    const int Ts = (tz * nnt + ty) * nnt + tx;
+    const int64_t s0 = (((int64_t)Ts * nte + (sz - 1 + ncb)) * nte + (sy - 1 + ncb)) * nte + (sx0 - 1 + ncb);
+    const int64_t src = eoff[s0], nsrc = eoff[s0 + (xe1 - xe0)] - src;
+    const int64_t dst = eoff[erow * nte + xe0], ndst = eoff[erow * nte + xe1] - dst;
+    const int64_t n = nsrc < ndst ? nsrc : ndst;
+    for (int64_t q = threadIdx.x; q < 3 * n; q += 256) vp[3 * dst + q] = vp[3 * src + q];
+    if (PID)
+      for (int64_t q = threadIdx.x; q < n; q += 256) pid[dst + q] = pid[src + q];
+  }
+}
+
 }  // namespace
+
+void buffer_v(const Geo& g, const int64_t* ext_off, float* vp, int64_t* pid, hipStream_t s) {
+  const int64_t rows = g.ntiles * g.nte * g.nte;
+  if (pid)
+    hipLaunchKernelGGL(k_buffer_v_rows<true>, dim3((unsigned)rows), dim3(256), 0, s, g, ext_off, vp, pid);
+  else
+    hipLaunchKernelGGL(k_buffer_v_rows<false>, dim3((unsigned)rows), dim3(256), 0, s, g, ext_off, vp, pid);
+  CUBE_HIP(hipGetLastError());
+}
 
 void buffer_copy(const Geo& g, const int32_t* rhoc_ext, const int64_t* ext_off, const int64_t* phys_off,
                  const void* xp, const float* vp, const int64_t* pid, void* xp_out, float* vp_out, int64_t* pid_out,

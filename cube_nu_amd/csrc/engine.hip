@@ -169,6 +169,7 @@ struct cube_p3m_ctx {
   void pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_in& in, PPOut* o);
   void drift(const cube_drift_in& in, cube_drift_out& out);
   void buffer(int64_t* np_ext);
+  void buffer_vel();
 };
 
 // ---------------------------------------------------------------------------
@@ -619,6 +620,14 @@ void cube_p3m_ctx::buffer(int64_t* np_ext) {
   if (np_ext) *np_ext = n;
 }
 
+// buffer_v at nn=1 on the resident extended layout.
+void cube_p3m_ctx::buffer_vel() {
+  if (layout != 1) throw Error(CUBE_ERR_STATE, "buffer_v needs the buffered (extended) particle layout");
+  ensure_offsets();
+  KScope k(&tm, "buffer_v", (double)np * 12);
+  buffer_v(g, off.p, vpc(), nullptr, s);
+}
+
 // ---------------------------------------------------------------------------
 // host spine (spine_image, variables.f90:225-295)
 static void host_spine(const Geo& g, const int32_t* rhoc, int64_t* idx_b_l, int64_t* idx_b_r, int64_t* ppl0,
@@ -858,6 +867,37 @@ int cube_p3m_buffer(cube_p3m_ctx* ctx, void* xp, float* vp, int64_t* pid, int32_
   });
 }
 
+int cube_p3m_buffer_v(cube_p3m_ctx* ctx, float* vp, int64_t* pid, const int32_t* rhoc, int64_t np) {
+  return guarded(ctx, [&] {
+    if (!vp || !rhoc) throw Error(CUBE_ERR_ARG, "null argument");
+    const Geo& g = ctx->g;
+    const int64_t n = host_sum(rhoc, g.ncells_ext);
+    if (n > np) throw Error(CUBE_ERR_ARG, "particle arrays shorter than sum(rhoc)");
+    ctx->ensure_set(0, std::max<int64_t>(n, 1));
+    ctx->cur = 0;
+    ctx->rcur = 0;
+    CUBE_HIP(hipMemcpyAsync(ctx->rhoc[0].p, rhoc, sizeof(int32_t) * g.ncells_ext, hipMemcpyHostToDevice, ctx->s));
+    CUBE_HIP(hipMemcpyAsync(ctx->vp[0].p, vp, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->s));
+    if (pid) CUBE_HIP(hipMemcpyAsync(ctx->pid[0].p, pid, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx->s));
+    ctx->layout = 1;
+    ctx->np = n;
+    ctx->off_valid = false;
+    ctx->ensure_offsets();
+    {
+      KScope k(&ctx->tm, "buffer_v", (double)n * 12);
+      buffer_v(g, ctx->off.p, ctx->vp[0].p, pid ? ctx->pid[0].p : nullptr, ctx->s);
+    }
+    CUBE_HIP(hipMemcpyAsync(vp, ctx->vp[0].p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->s));
+    if (pid) CUBE_HIP(hipMemcpyAsync(pid, ctx->pid[0].p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx->s));
+    CUBE_HIP(hipStreamSynchronize(ctx->s));
+    ctx->layout = 0;  // positions were not uploaded
+  });
+}
+
+int cube_p3m_buffer_v_resident(cube_p3m_ctx* ctx) {
+  return guarded(ctx, [&] { ctx->buffer_vel(); });
+}
+
 int cube_p3m_state_upload(cube_p3m_ctx* ctx, const void* xp, const float* vp, const int64_t* pid,
                           const int32_t* rhoc, int64_t np) {
   return guarded(ctx, [&] {
@@ -918,6 +958,7 @@ int cube_p3m_step_resident(cube_p3m_ctx* ctx, const cube_drift_in* din, const cu
     cube_kick_out k;
     if (ctx->prec == 32) ctx->kick<float>(*kin, k, nullptr);
     else ctx->kick<double>(*kin, k, nullptr);
+    ctx->buffer_vel();
     if (dout) *dout = d;
     if (kout) *kout = k;
   });

@@ -128,25 +128,51 @@ constexpr int kUnroll = 6;  // candidate loads in flight per lane (measured best
 static_assert(kUnroll <= kPPPad, "entries are padded for kUnroll reads past a run");
 
 
-// FP32 pair law F(r, app) - F(r, r_pp) for 0 < r < r_pp.  F(r, r_pp) is one
-// of the two polynomials of F_ra (kick.f90:360-374), both evaluated in Horner
-// form so that lanes do not diverge.  For r > app (all but vanishingly few
-// pairs) F(r, app) = 1/r^2, and the outer branch's c*12/ep^2 term, which
-// equals (3/35)/r^2 for c = (1/35)/r_pp^2 and ep = 2r/r_pp, is folded into
-// it: 1/r^2 - c (12/ep^2 + P(ep)) = (32/35)/r^2 - c P(ep).  The rare r <= app
-// pair takes the full forms.  The error of a pair term stays at the level of
-// FP32 rounding.
-__device__ inline float pair_law_fast(const PPLevel& L, float r2, float& rinv_out) {
+// F(r, r_pp) = c Q(ep), ep = 2r/r_pp < 2, with Q(ep) = pi(ep) for ep <= 1 and
+// 12/ep^2 + po(ep) above (the two polynomials of F_ra, kick.f90:360-374).
+// The FP32 force takes Q from a cubic Hermite table over kQSeg segments of
+// [0, 2] kept in LDS: ep = 1 is a segment edge and Q is C^1 there, so every
+// segment interpolates one smooth branch; the table's error is below 2e-5 on
+// |Q| <= 104 (h^4 max|Q^(4)| / 384 with h = 1/32), i.e. below 1e-6 of
+// 1/r_pp^2 in the pair force, at the level of FP32 rounding.
+constexpr int kQSeg = 64;
+
+__device__ inline void q_branch(double e, bool outer, double& q, double& dq) {
+  if (outer) {
+    q = 12.0 / (e * e) + (((((7.0 * e - 48.0) * e + 70.0) * e + 224.0) * e - 840.0) * e + 896.0) * e - 224.0;
+    dq = -24.0 / (e * e * e) + ((((42.0 * e - 240.0) * e + 280.0) * e + 672.0) * e - 1680.0) * e + 896.0;
+  } else {
+    q = e * (224.0 + e * e * (-224.0 + e * (70.0 + e * (48.0 - 21.0 * e))));
+    dq = 224.0 + e * e * (-672.0 + e * (280.0 + e * (240.0 - 126.0 * e)));
+  }
+}
+
+// Segment s of the table: Q(s h + f h) = ((c.w f + c.z) f + c.y) f + c.x.
+__device__ inline void fill_qtab(float4* tab) {
+  for (int sg = threadIdx.x; sg < kQSeg; sg += blockDim.x) {
+    const double h = 2.0 / kQSeg;
+    const bool outer = sg >= kQSeg / 2;
+    double y0, d0, y1, d1;
+    q_branch(sg * h, outer, y0, d0);
+    q_branch((sg + 1) * h, outer, y1, d1);
+    d0 *= h;
+    d1 *= h;
+    tab[sg] = make_float4((float)y0, (float)d0, (float)(3.0 * (y1 - y0) - 2.0 * d0 - d1),
+                          (float)(2.0 * (y0 - y1) + d0 + d1));
+  }
+}
+
+// FP32 pair law from the table: F(r, app) - F(r, r_pp) = 1/r^2 - c Q(ep) for
+// r > app; the rare r <= app pair takes F(r, app) in full.
+__device__ inline float pair_law_tab(const PPLevel& L, const float4* qtab, float r2, float& rinv_out) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * rinv;
-  const float rinv2 = rinv * rinv;
-  const float ep = r * L.two_inv_range;
-  const float e2 = ep * ep;
-  const float po =
-      fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, fmaf(ep, 7.0f, -48.0f), 70.0f), 224.0f), -840.0f), 896.0f), -224.0f);
-  const float pi = ep * fmaf(e2, fmaf(ep, fmaf(ep, fmaf(ep, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
-  const bool outer = ep > 1.0f;
-  float s = fmaf(outer ? 32.0f / 35.0f : 1.0f, rinv2, -L.c_range * (outer ? po : pi));
+  const float t = r * (L.two_inv_range * (0.5f * kQSeg));
+  const int i = min((int)t, kQSeg - 1);
+  const float f = t - (float)i;
+  const float4 c = qtab[i];
+  const float q = fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
+  float s = fmaf(-L.c_range, q, rinv * rinv);
   if (r2 <= L.app2) {  // ep(r, app) <= 2
     const float ea = (2.0f * r) * L.inv_app;
     const float a2 = ea * ea;
@@ -158,7 +184,7 @@ __device__ inline float pair_law_fast(const PPLevel& L, float r2, float& rinv_ou
                                        896.0f),
                               -224.0f))
             : L.c_app * ea * fmaf(a2, fmaf(ea, fmaf(ea, fmaf(ea, -21.0f, 48.0f), 70.0f), -224.0f), 224.0f);
-    s = fa - L.c_range * (outer ? 12.0f * __builtin_amdgcn_rcpf(e2) + po : pi);
+    s = fmaf(-L.c_range, q, fa);
   }
   rinv_out = rinv;
   return s;
@@ -166,20 +192,20 @@ __device__ inline float pair_law_fast(const PPLevel& L, float r2, float& rinv_ou
 
 // Pair force of candidate o on target me (kick.f90:523-530, 554-560):
 // f1 += (x2-x1)/r * (F(r,app) - F(r,r_pp)).
-// FP32 form: r^-1 from the hardware reciprocal square root, the F_ra
-// divisions replaced by products with precomputed reciprocals and the
-// polynomials in Horner form (pair_law_fast); the error of a pair term stays
-// at the level of FP32 rounding.  FP64 form: the reference's operations in
-// FP64.
+// FP32 form: r^-1 from the hardware reciprocal square root, F(r, app) = 1/r^2
+// (r > app) and F(r, r_pp) from the LDS table (pair_law_tab); the error of a
+// pair term stays at the level of FP32 rounding.  FP64 form: the reference's
+// operations in FP64.
 template <typename R>
-__device__ inline void pair_force(const PPLevel& L, const float4 me, const float4 o, R f1[3]);
+__device__ inline void pair_force(const PPLevel& L, const float4* qtab, const float4 me, const float4 o, R f1[3]);
 
 template <>
-__device__ inline void pair_force<float>(const PPLevel& L, const float4 me, const float4 o, float f1[3]) {
+__device__ inline void pair_force<float>(const PPLevel& L, const float4* qtab, const float4 me, const float4 o,
+                                         float f1[3]) {
   const float rv0 = o.x - me.x, rv1 = o.y - me.y, rv2 = o.z - me.z;
   const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;
   float rinv;
-  const float s = pair_law_fast(L, r2, rinv);
+  const float s = pair_law_tab(L, qtab, r2, rinv);
   const float sc = rinv * s;
   f1[0] = fmaf(rv0, sc, f1[0]);
   f1[1] = fmaf(rv1, sc, f1[1]);
@@ -187,7 +213,8 @@ __device__ inline void pair_force<float>(const PPLevel& L, const float4 me, cons
 }
 
 template <>
-__device__ inline void pair_force<double>(const PPLevel& L, const float4 me, const float4 o, double f1[3]) {
+__device__ inline void pair_force<double>(const PPLevel& L, const float4*, const float4 me, const float4 o,
+                                          double f1[3]) {
   const double rv0 = (double)o.x - me.x, rv1 = (double)o.y - me.y, rv2 = (double)o.z - me.z;
   const double r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;
   const double r = sqrt(r2);
@@ -209,7 +236,7 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4 me, con
 // here with the reference's FP32 operations and the interacting set is the
 // reference's.  The sub-bin starts of the (PB+2)^3 neighbourhood are staged
 // in LDS; candidates are read kUnroll at a time from L1/L2 (the entry array
-// is padded by kUnroll so the reads need no clamp).
+// is padded by kPPPad >= kUnroll entries so the reads need no clamp).
 template <typename R>
 __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   const int64_t* __restrict__ start,
@@ -220,6 +247,8 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const PPBox* __rest
   __shared__ int64_t rs[NB * NB * RW];
   __shared__ float red[4];
   __shared__ unsigned long long cnt_red[2];
+  __shared__ float4 qtab[kQSeg];
+  if (sizeof(R) == 4) fill_qtab(qtab);
   const int nbr = (L.npgrid + PB - 1) / PB;
   int64_t b = blockIdx.x;
   const int bx = (int)(b % nbr);
@@ -316,7 +345,7 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const PPBox* __rest
             const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
             const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
             if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
-              pair_force<R>(L, me, o[k], f1);
+              pair_force<R>(L, qtab, me, o[k], f1);
               ++npair;
               nghost += (unsigned)(p + k - ph0) >= phn;
             }

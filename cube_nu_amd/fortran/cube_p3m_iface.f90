@@ -148,6 +148,16 @@ module cube_p3m_iface
       integer(c_int64_t), intent(out) :: np_ext
     end function
 
+    ! buffer_v (buffer_particle.f90:11-27), after kick (main.f90:38)
+    integer(c_int) function cube_p3m_buffer_v(ctx, vp, pid, rhoc, np) bind(C, name='cube_p3m_buffer_v')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(inout) :: vp(*)
+      integer(c_int64_t), intent(inout) :: pid(*)
+      integer(c_int32_t), intent(in) :: rhoc(*)
+      integer(c_int64_t), value :: np
+    end function
+
     integer(c_int) function cube_p3m_cicpower(ctx, nw, xi, nbin, delta, mean) bind(C, name='cube_p3m_cicpower')
       import :: c_int, c_ptr, c_int32_t, c_float, c_double
       type(c_ptr), value :: ctx

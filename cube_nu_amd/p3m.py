@@ -98,7 +98,8 @@ class ICParams(C.Structure):
 # Names the C-ABI exports (include/cube_p3m.h).
 ABI_FUNCTIONS = (
     "cube_p3m_create", "cube_p3m_destroy", "cube_p3m_last_error", "cube_p3m_set_tf", "cube_p3m_get_tf",
-    "cube_p3m_particle_mesh", "cube_p3m_kick", "cube_p3m_drift", "cube_p3m_buffer", "cube_p3m_state_upload",
+    "cube_p3m_particle_mesh", "cube_p3m_kick", "cube_p3m_drift", "cube_p3m_buffer", "cube_p3m_buffer_v",
+    "cube_p3m_buffer_v_resident", "cube_p3m_state_upload",
     "cube_p3m_state_download", "cube_p3m_kick_resident", "cube_p3m_drift_resident", "cube_p3m_buffer_resident",
     "cube_p3m_step_resident", "cube_p3m_make_ic", "cube_p3m_cicpower", "cube_p3m_synchronize",
     "cube_p3m_timing_enable", "cube_p3m_timing_reset", "cube_p3m_timing_report", "cube_p3m_version",
@@ -142,6 +143,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cube_p3m_kick.argtypes = [P, vp_, vp_, vp_, vp_, i64, C.POINTER(KickIn), C.POINTER(KickOut)]
     lib.cube_p3m_drift.argtypes = [P, vp_, vp_, vp_, vp_, i64] + [vp_] * 8 + [C.POINTER(DriftIn), C.POINTER(DriftOut)]
     lib.cube_p3m_buffer.argtypes = [P, vp_, vp_, vp_, vp_, i64, i64, vp_, vp_, C.POINTER(C.c_int64)]
+    lib.cube_p3m_buffer_v.argtypes = [P, vp_, vp_, vp_, i64]
+    lib.cube_p3m_buffer_v_resident.argtypes = [P]
     lib.cube_p3m_state_upload.argtypes = [P, vp_, vp_, vp_, vp_, i64]
     lib.cube_p3m_state_download.argtypes = [P, vp_, vp_, vp_, vp_, i64, C.POINTER(C.c_int64)]
     lib.cube_p3m_kick_resident.argtypes = [P, C.POINTER(KickIn), C.POINTER(KickOut)]
@@ -370,6 +373,18 @@ class P3M:
         out = DriftOut()
         self._check(self.lib.cube_p3m_drift_resident(self.ctx, C.byref(din), C.byref(out)))
         return {"nplocal": out.nplocal, "overhead_tile": out.overhead_tile, "nlayer": out.nlayer}
+
+    def buffer_v(self, vp, rhoc, pid=None):
+        """buffer_v (buffer_particle.f90:11-27): buffer velocities (and IDs)
+        of the extended layout from their physical cells, in place."""
+        assert vp.dtype == np.float32 and vp.flags.c_contiguous and vp.shape[1] == 3
+        assert rhoc.dtype == np.int32 and rhoc.shape == self.geo.rhoc_shape and rhoc.flags.c_contiguous
+        if pid is not None:
+            assert pid.dtype == np.int64 and pid.shape == (vp.shape[0],) and pid.flags.c_contiguous
+        self._check(self.lib.cube_p3m_buffer_v(self.ctx, _ptr(vp), _ptr(pid), _ptr(rhoc), vp.shape[0]))
+
+    def buffer_v_resident(self):
+        self._check(self.lib.cube_p3m_buffer_v_resident(self.ctx))
 
     def buffer_resident(self):
         n = C.c_int64()

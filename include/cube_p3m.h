@@ -159,6 +159,15 @@ int cube_p3m_buffer(cube_p3m_ctx* ctx, void* xp, float* vp, int64_t* pid,
                     int32_t* rhoc, int64_t nplocal, int64_t np_cap,
                     int64_t* idx_b_l, int64_t* idx_b_r, int64_t* np_ext);
 
+/* buffer_v at nn=1 (buffer_particle.f90:11-27; called after kick,
+ * main.f90:38): in the extended layout of rhoc (np = length of the arrays),
+ * the velocities (and the IDs, when pid is not NULL) of every buffer cell are
+ * overwritten in place by those of the physical cell the buffer cell is the
+ * periodic image of, so that the next drift moves every copy of a particle
+ * alike.  Leaves no resident state behind. */
+int cube_p3m_buffer_v(cube_p3m_ctx* ctx, float* vp, int64_t* pid,
+                      const int32_t* rhoc, int64_t np);
+
 /* ---- resident mode: state kept on the device between calls ------------- */
 
 /* Upload the extended (buffered) layout. */
@@ -174,6 +183,9 @@ int cube_p3m_kick_resident(cube_p3m_ctx* ctx, const cube_kick_in* in,
 int cube_p3m_drift_resident(cube_p3m_ctx* ctx, const cube_drift_in* in,
                             cube_drift_out* out);
 int cube_p3m_buffer_resident(cube_p3m_ctx* ctx, int64_t* np_ext);
+/* buffer_v on the resident extended layout (velocities only: the buffer IDs
+ * written by the buffer step are already the physical cells' IDs). */
+int cube_p3m_buffer_v_resident(cube_p3m_ctx* ctx);
 /* One full step in the main.f90:33-38 order after timestep:
  * drift -> buffer_grid/buffer_x -> kick -> buffer_v. */
 int cube_p3m_step_resident(cube_p3m_ctx* ctx, const cube_drift_in* din,

@@ -1161,6 +1161,40 @@ int orc_buffer(const OParams* P, const void* xp, const float* vp, const int64_t*
   return 0;
 }
 
+// buffer_v at nn=1 (buffer_particle.f90:11-27 -> buffer_xvp(.false., .true.),
+// 29-375): in the extended layout, every buffer cell's velocities and IDs are
+// copied from the physical cell it is the periodic image of (the same cell
+// mapping as buffer_x); physical cells are untouched.  pid may be NULL.
+int orc_buffer_v(const OParams* P, float* vp, int64_t* pid, const int32_t* rhoc_ext, int64_t np) {
+  Geo g(*P);
+  const int64_t nt = g.nt, ncb = g.ncb, nnt = g.nnt, nc = g.nc, nte = nt + 2 * ncb;
+  // extended offsets in storage order
+  const int64_t ncell = g.ntiles() * nte * nte * nte;
+  std::vector<int64_t> eoff(ncell + 1, 0);
+  for (int64_t c = 0; c < ncell; ++c) eoff[c + 1] = eoff[c] + rhoc_ext[c];
+  if (eoff[ncell] > np) return 4;
+  for (int64_t T = 0; T < g.ntiles(); ++T) {
+    int64_t it[3] = {T % nnt, (T / nnt) % nnt, T / (nnt * nnt)};
+    for (int64_t k = 1 - ncb; k <= nt + ncb; ++k)
+      for (int64_t j = 1 - ncb; j <= nt + ncb; ++j)
+        for (int64_t i = 1 - ncb; i <= nt + ncb; ++i) {
+          if (i >= 1 && i <= nt && j >= 1 && j <= nt && k >= 1 && k <= nt) continue;
+          int64_t ijk[3] = {i, j, k}, st[3], sc[3];
+          for (int d = 0; d < 3; ++d) {
+            int64_t gc = it[d] * nt + ijk[d] - 1;
+            gc = ((gc % nc) + nc) % nc;
+            st[d] = gc / nt; sc[d] = gc % nt + 1;
+          }
+          const int64_t Ts = (st[2] * nnt + st[1]) * nnt + st[0];
+          const int64_t cs = g.rc(sc[0], sc[1], sc[2], Ts), cd = g.rc(i, j, k, T);
+          if (rhoc_ext[cs] != rhoc_ext[cd]) return 8;  // not a buffered layout
+          std::memcpy(vp + 3 * eoff[cd], vp + 3 * eoff[cs], sizeof(float) * 3 * rhoc_ext[cd]);
+          if (pid) std::memcpy(pid + eoff[cd], pid + eoff[cs], sizeof(int64_t) * rhoc_ext[cd]);
+        }
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // CIC power spectrum of the physical particles (utilities/cicpower.f90:67-179
 // with utilities/powerspectrum.f90:140-259 auto_power and pk_correction),

@@ -232,6 +232,13 @@ def buffer(P, xp, vp, pid, rhoc_phys, cap):
     return xo[:k], vo[:k], po[:k], ro
 
 
+def buffer_v(P, vp, rhoc_ext, pid=None):
+    """buffer_v of an extended layout, in place on vp (and pid)."""
+    st = lib().orc_buffer_v(C.byref(P), _p(vp), _p(pid), _p(rhoc_ext), C.c_int64(vp.shape[0]))
+    if st != 0:
+        raise RuntimeError(f"orc_buffer_v status {st}")
+
+
 def cicpower(P, xp, rhoc_phys, nw, box, want_delta=False):
     """CIC P(k) of a physical layout: (xi (nbin+1, 10), mean, delta or None)."""
     nyq = nw // 2

@@ -11,7 +11,7 @@ import pytest
 
 from cube_nu_amd import P3M
 from cube_nu_amd.p3m import CubeError, CUBE_ERR_OUT_OF_RANGE
-from inputs import buffered_set, oparams, physical_set, rel_rms
+from inputs import buffered_set, oparams, physical_mask, physical_set, rel_rms
 from oracle import pyoracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -136,6 +136,53 @@ def test_buffer_bit_exact(gpu):
     assert m == xe.shape[0]
     assert np.array_equal(rg, re)
     assert np.array_equal(xg[:m], xe) and np.array_equal(vg[:m], ve) and np.array_equal(pg[:m], pe)
+
+
+def test_buffer_v_bit_exact(gpu):
+    geo = _geo()
+    xp, vp, pid, rhoc = physical_set(geo, seed=6, mean=5.0)
+    cap = int(xp.shape[0] * (geo.nte / geo.nt) ** 3 * 1.2) + 1024
+    v_new = (vp * np.float32(1.5) + np.float32(0.25)).astype(np.float32)
+    xe, ve, pe, re = orc.buffer(oparams(geo), xp, v_new, pid, rhoc, cap)
+    phys = physical_mask(geo, re)
+    v = ve.copy()
+    p = pe.copy()
+    v[~phys] = -7.0
+    p[~phys] = 0
+    gpu.buffer_v(v, re, p)
+    assert np.array_equal(v, ve) and np.array_equal(p, pe)
+    v = ve.copy()
+    v[~phys] = 2.0
+    gpu.buffer_v(v, re)
+    assert np.array_equal(v, ve)
+
+
+def test_resident_steps_conserve_particles():
+    """Several resident steps (drift, buffer, kick, buffer_v) of a clustered
+    Zel'dovich start keep every particle exactly once: nplocal and the PID
+    set are preserved, and the buffered layout's buffer cells carry their
+    physical cells' velocities after every step."""
+    p = P3M(NG, izipx=2, precision=32, box=600.0 * NG / 512)
+    try:
+        ic = p.make_ic(seed=7, z=1.0, nic=2)
+        a = 0.5
+        k0 = p.kick_resident(a_mid=a, dt=0.0)
+        dt = min(k0["dt_pm1"], k0["dt_pm2"], k0["dt_pm3"], k0["dt_pp"], 1.0) * 2
+        vz = ic["vz_max"]
+        for _ in range(4):
+            d, k = p.step_resident(dt_old=dt, dt=dt, vz_max=vz, a_mid=a)
+            vz = k["vz_max"]
+            assert d["nplocal"] == NG ** 3
+        xe, ve, pe, re = p.download()
+        v = ve.copy()
+        orc.buffer_v(oparams(_geo()), v, re)
+        assert np.array_equal(v, ve)
+        p.drift_resident(dt_old=dt, dt=dt, vz_max=vz)
+        xp, vp, pid, rhoc = p.download()
+        assert xp.shape[0] == NG ** 3
+        assert np.array_equal(np.sort(pid), np.arange(1, NG ** 3 + 1))
+    finally:
+        p.close()
 
 
 def test_drift_out_of_range_error(gpu, data):

@@ -236,6 +236,31 @@ def test_buffer_is_the_periodic_image():
     assert np.array_equal(pe, pid[idx]) and np.array_equal(xe, xp[idx]) and np.array_equal(ve, vp[idx])
 
 
+def test_buffer_v_restores_the_periodic_copies():
+    """buffer_v on a buffered layout whose physical velocities changed (as
+    after a kick) and whose buffer velocities and IDs are stale gives exactly
+    the buffer step's layout of the new physical velocities; physical cells
+    are not touched."""
+    geo = Geometry(64)
+    xp, vp, pid, rhoc = physical_set(geo, seed=21, mean=3.0)
+    P = oparams(geo)
+    cap = int(xp.shape[0] * (geo.nte / geo.nt) ** 3 * 1.2) + 1024
+    v_new = (vp + np.random.default_rng(3).normal(size=vp.shape)).astype(np.float32)
+    xe, ve, pe, re = orc.buffer(P, xp, v_new, pid, rhoc, cap)
+    phys = physical_mask(geo, re)
+    v = ve.copy()
+    p = pe.copy()
+    v[~phys] = -7.0
+    p[~phys] = 0
+    orc.buffer_v(P, v, re, p)
+    assert np.array_equal(v, ve) and np.array_equal(p, pe)
+    # velocities only
+    v = ve.copy()
+    v[~phys] = 3.0
+    orc.buffer_v(P, v, re)
+    assert np.array_equal(v, ve)
+
+
 # ---------------------------------------------------------------------- drift
 def _drift_spec(geo, xp, vp, pid, rhoc, dt_old, dt, vz_max):
     """Drift from the written spec (drift.f90:21-136): destination cell,

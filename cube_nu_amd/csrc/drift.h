@@ -28,14 +28,17 @@ struct DriftScratch {
 };
 
 // Error bits written to flags[0].
-enum { DRIFT_OUT_OF_RANGE = 1, DRIFT_PID_ZERO = 2 };
+enum { DRIFT_OUT_OF_RANGE = 1, DRIFT_PID_ZERO = 2, DRIFT_SHIFT_BOUND = 8 };
 
 // Pass 1: destination range check and maximum shift.
 void drift_shift(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
                  int64_t ncells, DriftScratch& S, hipStream_t s);
 // Passes 2-5 for tiles [t0, t1): rank, offsets, scatter (base scan done by caller).
+// count = true folds pass 1 into the rank pass for all tiles at once, with
+// P.M an upper bound of the cell shift given by the caller (flags as pass 1,
+// plus DRIFT_SHIFT_BOUND when a shift exceeds P.M).
 void drift_rank(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
-                int t0, int t1, DriftScratch& S, hipStream_t s);
+                int t0, int t1, DriftScratch& S, hipStream_t s, bool count = false);
 void drift_offsets(const DriftParams& P, int t0, int t1, DriftScratch& S, hipStream_t s);
 void drift_scatter(const DriftParams& P, const void* xp, const float* vp, const int64_t* pid, const int32_t* rhoc,
                    const int64_t* off, int t0, int t1, DriftScratch& S, void* xp_out, float* vp_out,

@@ -111,10 +111,16 @@ __global__ void k_drift_shift(DriftParams P, const XT<IZ>* __restrict__ xp, cons
   }
 }
 
-template <int IZ>
+// Rank pass.  COUNT = true also does the shift pass's work in the same read
+// of the particles (range check, destination counts, largest shift), with
+// P.M a bound given by the caller: a particle whose shift exceeds it takes no
+// table entry and sets DRIFT_SHIFT_BOUND, after which the caller redoes the
+// drift with the measured bound.
+template <int IZ, bool COUNT>
 __global__ void k_drift_rank(DriftParams P, const XT<IZ>* __restrict__ xp, const float* __restrict__ vp,
                              const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off, int64_t c0,
-                             int64_t ncells, int32_t* __restrict__ rank, int32_t* __restrict__ table) {
+                             int64_t ncells, int32_t* __restrict__ rank, int32_t* __restrict__ table,
+                             int32_t* __restrict__ flags, int32_t* __restrict__ newcount) {
   const int64_t cl = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (cl >= ncells) return;
@@ -125,13 +131,42 @@ __global__ void k_drift_rank(DriftParams P, const XT<IZ>* __restrict__ xp, const
   const int64_t base = off[c];
   const int Dt = P.D[0] * P.D[1] * P.D[2];
   int32_t* tbl = table + cl * Dt;
+  const int lo = 1 - 2 * P.ncb, hi = P.nt + 2 * P.ncb;
+  int err = 0, ms[3] = {0, 0, 0};
   for (int l0 = 0; l0 < cnt; l0 += 64) {
     const int l = l0 + lane;
     int key = -1;
+    int64_t pkey = -1;
     if (l < cnt) {
       int g[3];
       dest_cell<IZ>(P, xp, vp, base + l, ijk, g);
-      if (physical(P, g)) key = dcode(P, g, ijk);
+      if (COUNT)
+        for (int d = 0; d < 3; ++d)
+          if (g[d] < lo || g[d] > hi) err |= DRIFT_OUT_OF_RANGE;
+      if (physical(P, g)) {
+        bool fits = true;
+        if (COUNT) {
+          for (int d = 0; d < 3; ++d) {
+            const int sd = abs(g[d] - ijk[d]);
+            ms[d] = max(ms[d], sd);
+            fits = fits && sd <= P.M[d];
+          }
+          pkey = pcell(P, T, g);
+        }
+        if (fits) key = dcode(P, g, ijk);
+        else err |= DRIFT_SHIFT_BOUND;
+      }
+    }
+    if (COUNT) {
+      // one integer atomic per distinct destination in the wavefront
+      unsigned long long act = __ballot(pkey >= 0);
+      while (act) {
+        const int leader = __ffsll((long long)act) - 1;
+        const int64_t lk = __shfl(pkey, leader, 64);
+        const unsigned long long peers = __ballot(pkey == lk);
+        if (lane == leader) atomicAdd(&newcount[lk], (int)__popcll(peers));
+        act &= ~peers;
+      }
     }
     unsigned long long active = __ballot(key >= 0);
     int myrank = -1;
@@ -147,6 +182,16 @@ __global__ void k_drift_rank(DriftParams P, const XT<IZ>* __restrict__ xp, const
       active &= ~peers;
     }
     if (l < cnt) rank[base + l] = myrank;
+  }
+  if (COUNT) {
+    for (int o = 32; o > 0; o >>= 1) {
+      err |= __shfl_down(err, o, 64);
+      for (int d = 0; d < 3; ++d) ms[d] = max(ms[d], __shfl_down(ms[d], o, 64));
+    }
+    if (lane == 0 && cnt > 0) {
+      if (err) atomicOr(&flags[0], err);
+      for (int d = 0; d < 3; ++d) atomic_max_int(&flags[1 + d], ms[d]);
+    }
   }
 }
 
@@ -278,16 +323,26 @@ void drift_shift(const DriftParams& P, const void* xp, const float* vp, const in
 }
 
 void drift_rank(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
-                int t0, int t1, DriftScratch& S, hipStream_t s) {
+                int t0, int t1, DriftScratch& S, hipStream_t s, bool count) {
   const int64_t nct = (int64_t)P.nte * P.nte * P.nte;
   const int64_t ncells = (int64_t)(t1 - t0) * nct;
-  const int64_t nthr = ncells * 64;
-  if (P.izipx == 2)
-    hipLaunchKernelGGL(k_drift_rank<2>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P, (const int16_t*)xp,
-                       vp, rhoc, off, (int64_t)t0 * nct, ncells, S.rank, S.table);
-  else
-    hipLaunchKernelGGL(k_drift_rank<1>, dim3((unsigned)ceil_div(nthr, 256)), dim3(256), 0, s, P, (const int8_t*)xp,
-                       vp, rhoc, off, (int64_t)t0 * nct, ncells, S.rank, S.table);
+  const dim3 grid((unsigned)ceil_div(ncells * 64, 256));
+  const int64_t c0 = (int64_t)t0 * nct;
+  if (P.izipx == 2) {
+    if (count)
+      hipLaunchKernelGGL((k_drift_rank<2, true>), grid, dim3(256), 0, s, P, (const int16_t*)xp, vp, rhoc, off, c0,
+                         ncells, S.rank, S.table, S.flags, S.newcount);
+    else
+      hipLaunchKernelGGL((k_drift_rank<2, false>), grid, dim3(256), 0, s, P, (const int16_t*)xp, vp, rhoc, off, c0,
+                         ncells, S.rank, S.table, S.flags, S.newcount);
+  } else {
+    if (count)
+      hipLaunchKernelGGL((k_drift_rank<1, true>), grid, dim3(256), 0, s, P, (const int8_t*)xp, vp, rhoc, off, c0,
+                         ncells, S.rank, S.table, S.flags, S.newcount);
+    else
+      hipLaunchKernelGGL((k_drift_rank<1, false>), grid, dim3(256), 0, s, P, (const int8_t*)xp, vp, rhoc, off, c0,
+                         ncells, S.rank, S.table, S.flags, S.newcount);
+  }
   CUBE_HIP(hipGetLastError());
 }
 

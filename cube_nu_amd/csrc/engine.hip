@@ -82,6 +82,10 @@ struct cube_p3m_ctx {
   int cur = 0;
 
   int layout = 0;  // 0 none, 1 extended (buffered), 2 physical (after drift)
+  // max |v| per axis after the last full resident kick (kick.f90:299-313): a
+  // bound of every resident velocity until the next upload
+  float last_vmax[3] = {0, 0, 0};
+  bool vmax_valid = false;
   int64_t np = 0;
   bool has_pid = false;
   DevBuf<int32_t> rhoc[2];
@@ -491,6 +495,9 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
   }
   for (int d = 0; d < 3; ++d) out.vmax[d] = vmax[d];
   out.dt_vmax = (0.9f * 14.0f) / std::max({vmax[0], vmax[1], vmax[2]});  // kick.f90:314
+  // every physical particle took the last phase's kick, whose vmax covers it
+  vmax_valid = mask == 15;
+  for (int d = 0; d < 3; ++d) last_vmax[d] = vmax[d];
   out.vz_max = vmax[2];                                                     // kick.f90:315
 }
 
@@ -524,51 +531,97 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
   S.flags = dflags.p;
   S.newcount = dnewcount.ensure(nphys_cells);
   S.base = dbase.ensure(nphys_cells + 1);
-  zero_async(S.flags, sizeof(int32_t) * 8, s);
-  zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
-  {
-    KScope k(&tm, "drift_shift", (double)np * (3 * esz() + 12));
-    drift_shift(P, xpc(), vpc(), rhocc(), off.p, g.ncells_ext, S, s);
-  }
-  int32_t fl[4];
-  CUBE_HIP(hipMemcpyAsync(fl, S.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
-  {
-    KScope k(&tm, "drift_scan", (double)nphys_cells * 12);
-    scan(S.newcount, S.base, nphys_cells);
-  }
-  int64_t nplocal = 0;
-  CUBE_HIP(hipMemcpyAsync(&nplocal, S.base + nphys_cells, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CUBE_HIP(hipStreamSynchronize(s));
-  if (fl[0] & DRIFT_OUT_OF_RANGE)
-    throw Error(CUBE_ERR_OUT_OF_RANGE, "error: particle out of range (drift beyond 2*ncb coarse cells)");
-  for (int d = 0; d < 3; ++d) {
-    if (fl[1 + d] > kMaxShift) throw Error(CUBE_ERR_OUT_OF_RANGE, "drift shift beyond 16 coarse cells");
-    P.M[d] = fl[1 + d];
-    P.D[d] = 2 * P.M[d] + 1;
-  }
-  ensure_set(1 - cur, nplocal);
-  const int64_t Dt = (int64_t)P.D[0] * P.D[1] * P.D[2];
-  const int64_t per_tile = g.ncell_tile * Dt;
-  const int64_t tbudget = std::max<int64_t>(per_tile, ((int64_t)2 << 30) / 4);
-  const int tg = (int)std::max<int64_t>(1, std::min<int64_t>(g.ntiles, tbudget / per_tile));
-  S.table = dtable.ensure((size_t)(tg * per_tile));
-  S.rank = drank.ensure((size_t)std::max<int64_t>(np, 1));
   const int other = 1 - cur;
-  for (int t0 = 0; t0 < (int)g.ntiles; t0 += tg) {
-    const int t1 = std::min<int>((int)g.ntiles, t0 + tg);
-    zero_async(S.table, sizeof(int32_t) * (size_t)(t1 - t0) * per_tile, s);
+  int32_t fl[4];
+  int64_t nplocal = 0;
+  auto table_tiles = [&](int64_t Dt) {  // tiles per rank/offset batch within a 2 GiB table
+    const int64_t per_tile = g.ncell_tile * Dt;
+    const int64_t tbudget = std::max<int64_t>(per_tile, ((int64_t)2 << 30) / 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g.ntiles, tbudget / per_tile));
+  };
+  auto count_scan = [&]() {  // flags and the scan of the destination counts
+    CUBE_HIP(hipMemcpyAsync(fl, S.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
     {
-      KScope k(&tm, "drift_rank", (double)np * (3 * esz() + 16) * (t1 - t0) / g.ntiles);
-      drift_rank(P, xpc(), vpc(), rhocc(), off.p, t0, t1, S, s);
+      KScope k(&tm, "drift_scan", (double)nphys_cells * 12);
+      scan(S.newcount, S.base, nphys_cells);
     }
+    CUBE_HIP(hipMemcpyAsync(&nplocal, S.base + nphys_cells, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CUBE_HIP(hipStreamSynchronize(s));
+    if (fl[0] & DRIFT_OUT_OF_RANGE)
+      throw Error(CUBE_ERR_OUT_OF_RANGE, "error: particle out of range (drift beyond 2*ncb coarse cells)");
+  };
+  auto offsets_scatter = [&](int t0, int t1) {
     {
-      KScope k(&tm, "drift_offsets", (double)(t1 - t0) * nt3 * Dt * 8);
+      KScope k(&tm, "drift_offsets", (double)(t1 - t0) * nt3 * P.D[0] * P.D[1] * P.D[2] * 8);
       drift_offsets(P, t0, t1, S, s);
     }
     {
       KScope k(&tm, "drift_scatter", (double)np * 2 * (3 * esz() + 20) * (t1 - t0) / g.ntiles);
       drift_scatter(P, xpc(), vpc(), pidc(), rhocc(), off.p, t0, t1, S, xp[other].p, vp[other].p,
                     has_pid ? pid[other].p : nullptr, s);
+    }
+  };
+  S.rank = drank.ensure((size_t)std::max<int64_t>(np, 1));
+  // After a full resident kick the cell shift per axis is bounded by
+  // ceil(fl32(fl32(dt_mid vmax)/rcs)) (a particle in cell i has xq in (i-1, i)
+  // and moves by fl32(fl32(dt_mid v)/rcs), monotone in |v|), so the range
+  // check and the destination counts ride on the rank pass's one read of the
+  // particles; a shift beyond the bound (stale velocities) falls back to the
+  // separate pass.
+  bool fused = vmax_valid;
+  int Mb[3] = {0, 0, 0};
+  for (int d = 0; d < 3 && fused; ++d) {
+    const float dm = std::fabs((P.dt_mid * last_vmax[d]) / (float)g.rcs);
+    if (!(dm <= (float)kMaxShift)) fused = false;
+    else Mb[d] = (int)std::ceil(dm);
+  }
+  if (fused) fused = table_tiles((int64_t)(2 * Mb[0] + 1) * (2 * Mb[1] + 1) * (2 * Mb[2] + 1)) == (int)g.ntiles;
+  if (fused) {
+    for (int d = 0; d < 3; ++d) {
+      P.M[d] = Mb[d];
+      P.D[d] = 2 * Mb[d] + 1;
+    }
+    const int64_t Dt = (int64_t)P.D[0] * P.D[1] * P.D[2];
+    S.table = dtable.ensure((size_t)(g.ntiles * g.ncell_tile * Dt));
+    zero_async(S.flags, sizeof(int32_t) * 8, s);
+    zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
+    zero_async(S.table, sizeof(int32_t) * (size_t)(g.ntiles * g.ncell_tile * Dt), s);
+    {
+      KScope k(&tm, "drift_rank", (double)np * (3 * esz() + 16));
+      drift_rank(P, xpc(), vpc(), rhocc(), off.p, 0, (int)g.ntiles, S, s, true);
+    }
+    count_scan();
+    if (fl[0] & DRIFT_SHIFT_BOUND) fused = false;
+    else {
+      ensure_set(other, nplocal);
+      offsets_scatter(0, (int)g.ntiles);
+    }
+  }
+  if (!fused) {
+    zero_async(S.flags, sizeof(int32_t) * 8, s);
+    zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
+    {
+      KScope k(&tm, "drift_shift", (double)np * (3 * esz() + 12));
+      drift_shift(P, xpc(), vpc(), rhocc(), off.p, g.ncells_ext, S, s);
+    }
+    count_scan();
+    for (int d = 0; d < 3; ++d) {
+      if (fl[1 + d] > kMaxShift) throw Error(CUBE_ERR_OUT_OF_RANGE, "drift shift beyond 16 coarse cells");
+      P.M[d] = fl[1 + d];
+      P.D[d] = 2 * P.M[d] + 1;
+    }
+    ensure_set(other, nplocal);
+    const int64_t Dt = (int64_t)P.D[0] * P.D[1] * P.D[2];
+    const int tg = table_tiles(Dt);
+    S.table = dtable.ensure((size_t)(tg * g.ncell_tile * Dt));
+    for (int t0 = 0; t0 < (int)g.ntiles; t0 += tg) {
+      const int t1 = std::min<int>((int)g.ntiles, t0 + tg);
+      zero_async(S.table, sizeof(int32_t) * (size_t)(t1 - t0) * g.ncell_tile * Dt, s);
+      {
+        KScope k(&tm, "drift_rank", (double)np * (3 * esz() + 16) * (t1 - t0) / g.ntiles);
+        drift_rank(P, xpc(), vpc(), rhocc(), off.p, t0, t1, S, s);
+      }
+      offsets_scatter(t0, t1);
     }
   }
   int32_t* rnew = rhoc[1 - rcur].p;
@@ -705,6 +758,7 @@ static void upload_ext(cube_p3m_ctx* c, const void* xp, const float* vp, const i
   c->layout = 1;
   c->np = n;
   c->off_valid = false;
+  c->vmax_valid = false;
 }
 
 extern "C" {
@@ -852,6 +906,7 @@ int cube_p3m_buffer(cube_p3m_ctx* ctx, void* xp, float* vp, int64_t* pid, int32_
     CUBE_HIP(hipMemcpyAsync(ctx->vp[0].p, vp, sizeof(float) * 3 * nplocal, hipMemcpyHostToDevice, ctx->s));
     ctx->has_pid = pid != nullptr;
     if (pid) CUBE_HIP(hipMemcpyAsync(ctx->pid[0].p, pid, sizeof(int64_t) * nplocal, hipMemcpyHostToDevice, ctx->s));
+    ctx->vmax_valid = false;
     ctx->layout = 2;
     ctx->np = nplocal;
     int64_t n = 0;
@@ -891,6 +946,7 @@ int cube_p3m_buffer_v(cube_p3m_ctx* ctx, float* vp, int64_t* pid, const int32_t*
     if (pid) CUBE_HIP(hipMemcpyAsync(pid, ctx->pid[0].p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx->s));
     CUBE_HIP(hipStreamSynchronize(ctx->s));
     ctx->layout = 0;  // positions were not uploaded
+    ctx->vmax_valid = false;
   });
 }
 
@@ -975,6 +1031,7 @@ int cube_p3m_make_ic(cube_p3m_ctx* ctx, const cube_ic_params* ic, float* vz_max,
     ctx->layout = 2;
     ctx->np = r.nplocal;
     ctx->off_valid = false;
+    ctx->vmax_valid = false;
     int64_t n = 0;
     ctx->buffer(&n);
     if (vz_max) *vz_max = r.vz_max;

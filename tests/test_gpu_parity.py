@@ -185,6 +185,32 @@ def test_resident_steps_conserve_particles():
         p.close()
 
 
+def test_resident_drift_after_kick_bit_exact():
+    """A resident drift right after a full kick and buffer_v takes the fused
+    count + rank pass (its cell-shift bound comes from the kick's max |v|)
+    and gives the oracle's drift bit for bit."""
+    geo = _geo()
+    p = P3M(NG, izipx=2, precision=32)
+    try:
+        xp, vp, pid, rhoc = buffered_set(geo, seed=13, mean=6.0)
+        p.upload(xp, vp, pid, rhoc)
+        k = p.kick_resident(a_mid=A_MID, dt=DT)
+        p.buffer_v_resident()
+        xe, ve, pe, re = p.download()
+        p.timing(True)
+        p.timing_reset()
+        d = p.drift_resident(dt_old=0.4, dt=0.6, vz_max=k["vz_max"])
+        rep = p.timing_report()
+        assert rep.get("drift_shift", {"calls": 0})["calls"] == 0 and rep["drift_rank"]["calls"] == 1
+        xg, vg, pg, rg = p.download()
+        st, xo, vo, po, ro, oo = orc.drift(oparams(geo), xe, ve, pe, re, 0.4, 0.6, k["vz_max"])
+        assert st == 0 and d["nplocal"] == oo["nplocal"] and d["nlayer"] == oo["nlayer"]
+        assert np.array_equal(rg, ro)
+        assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(pg, po)
+    finally:
+        p.close()
+
+
 def test_drift_out_of_range_error(gpu, data):
     xp, vp, pid, rhoc = data
     v = vp.copy()

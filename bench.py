@@ -63,9 +63,10 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
     p.synchronize()
     t = time.perf_counter() - t0
     rep = p.timing_report() if timing else {}
+    tuned = {k: v["calls"] for k, v in p.timing_report().items() if k.startswith("pp_unroll_level")}
     p.close()
     log(f"[bench] fp{precision}: {steps} steps in {t:.3f}s ({1e3 * t / steps:.1f} ms/step)")
-    return {"seconds": t, "n": nloc, "report": rep, "dt": dt}
+    return {"seconds": t, "n": nloc, "report": rep, "dt": dt, "autotune": tuned}
 
 
 def measured_traffic(kernel, ng):
@@ -392,7 +393,8 @@ def main():
                                "(drift + buffer + PM1/PM2/PM3/PP kick), int16 coordinates",
                    "ng": args.ng, "izipx": 2, "dt": r32["dt"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64_form": fp64, "fp64_plain_per_tile": fp64_plain,
-        "phase_ms_per_step": phases, "pm_forward_adjoint": pm_adj, "fp32_vs_fp64": prec,
+        "phase_ms_per_step": phases, "autotune": r32["autotune"], "pm_forward_adjoint": pm_adj,
+        "fp32_vs_fp64": prec,
     }
     print(json.dumps(line), flush=True)
 

@@ -101,7 +101,7 @@ struct cube_p3m_ctx {
   DevBuf<PPBox> ppboxes;
   DevBuf<PMOut> pmout;
   DevBuf<PPOut> ppout;
-  DevBuf<int32_t> ires, napm;
+  DevBuf<int32_t> ires, napm, cmax;
   DevBuf<int32_t> depcount;
   DevBuf<int64_t> depstart;
   DevBuf<unsigned long long> depents;
@@ -113,6 +113,10 @@ struct cube_p3m_ctx {
   DevBuf<int32_t> physcount;
   DevBuf<int64_t> physoff;
   int64_t last_pairs = 0;
+  // autotuned PP load depth per level (0 = not tuned yet) and its scratch
+  int pp_unroll[6] = {0, 0, 0, 0, 0, 0};
+  DevBuf<float> vtune;
+  DevBuf<PPOut> otune;
 
   ~cube_p3m_ctx() {
     for (int i = 0; i < 7; ++i) {
@@ -170,7 +174,9 @@ struct cube_p3m_ctx {
   template <typename R>
   void run_level(int level, const std::vector<PMBox>& bx, const float* G, const FFTCube<R>& fc, const PMLevel& L,
                  const cube_kick_in& in, PMOut* o, const char* tag, float* density_host = nullptr);
-  void pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_in& in, PPOut* o);
+  void pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h, const cube_kick_in& in,
+                PPOut* o);
+  int tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S);
   void drift(const cube_drift_in& in, cube_drift_out& out);
   void buffer(int64_t* np_ext);
   void buffer_vel();
@@ -203,6 +209,7 @@ void cube_p3m_ctx::init() {
   ppout.ensure(1);
   napm.ensure(8);
   ires.ensure(g.ns3);
+  cmax.ensure(g.ns3);
   dflags.ensure(8);
   set_tf_default();
 }
@@ -285,7 +292,8 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
   }
 }
 
-void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_in& in, PPOut* o) {
+void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h,
+                            const cube_kick_in& in, PPOut* o) {
   // mass_scale and v_scale as kick.f90:569-574
   float mass_scale;
   if (cfg.pp_corr && cfg.mass_nu > 0)
@@ -294,9 +302,16 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
     mass_scale = in.mass_p_cdm;
   const float v_scale = (float)((double)((in.a_mid * in.dt) / 6.0f) / kPi);
   const int64_t nns = g.nns;
+  // sub-tiles by descending occupancy (kick.f90:132-138 sorts them by npc_max
+  // for schedule(dynamic,1)): the workgroups of the densest sub-tiles are
+  // dispatched first, so the launch does not end on a few heavy ones; each
+  // sub-tile's result does not depend on the order
+  std::vector<int64_t> order(g.ns3);
+  for (int64_t it = 0; it < g.ns3; ++it) order[it] = it;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cmax_h[a] > cmax_h[b]; });
   for (int iapm = 2; iapm <= 5; ++iapm) {
     std::vector<PPBox> bx;
-    for (int64_t it = 0; it < g.ns3; ++it)
+    for (int64_t it : order)
       if (ires_h[it] == iapm) {
         PPBox b;
         b.ifs[0] = (int)(it % nns);
@@ -358,12 +373,50 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const cube_kick_
         KScope k(&tm, "pp_bin_fill", (double)total * 16);
         pp_bin_fill((int)g.izipx, L, ppboxes.p, nb, xpc(), rhocc(), off.p, S, s);
       }
+      const int unroll = prec == 32 ? tune_pp(iapm, L, nb, S) : 6;
       {
         KScope k(&tm, "pp_force", (double)total * 16);
-        pp_force(L, prec, ppboxes.p, nb, vpc(), S, o, s);
+        pp_force(L, prec, ppboxes.p, nb, vpc(), S, o, s, unroll);
       }
     }
   }
+}
+
+// Autotuner of the FP32 PP kernel's load depth (candidates in flight per
+// lane).  On the first batch of each PP level, every depth runs twice on a
+// scratch copy of the velocities and of the outputs, timed with HIP events on
+// the library stream, and the fastest is kept for the rest of the context's
+// life.  All depths sum the pairs in the same order, so the choice changes
+// the time of the step only.
+int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
+  if (pp_unroll[iapm]) return pp_unroll[iapm];
+  static const int cand[3] = {4, 6, 8};
+  vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
+  otune.ensure(1);
+  hipEvent_t e0, e1;
+  CUBE_HIP(hipEventCreate(&e0));
+  CUBE_HIP(hipEventCreate(&e1));
+  float best = 0;
+  int pick = 6;
+  for (int rep = 0; rep < 2; ++rep)
+    for (int u : cand) {
+      CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
+      zero_async(otune.p, sizeof(PPOut), s);
+      CUBE_HIP(hipEventRecord(e0, s));
+      pp_force(L, 32, ppboxes.p, nb, vtune.p, S, otune.p, s, u);
+      CUBE_HIP(hipEventRecord(e1, s));
+      CUBE_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      CUBE_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (best == 0 || ms < best) {
+        best = ms;
+        pick = u;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pp_unroll[iapm] = pick;
+  return pick;
 }
 
 template <typename R>
@@ -414,10 +467,11 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
     run_level<R>(2, bx, tf[2].p, fc(2), L, in, pmout.p + 1, "pm2_");
   }
   // classification (kick.f90:108-138)
-  std::vector<int32_t> ires_h(g.ns3, 2);
+  std::vector<int32_t> ires_h(g.ns3, 2), cmax_h(g.ns3, 0);
   if (mask & 12) {
-    classify_subtiles(g, rhocc(), ires.p, napm.p, s);
+    classify_subtiles(g, rhocc(), ires.p, napm.p, cmax.p, s);
     CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
+    CUBE_HIP(hipMemcpyAsync(cmax_h.data(), cmax.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipMemcpyAsync(out.napm, napm.p, sizeof(int32_t) * 7, hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipStreamSynchronize(s));
   }
@@ -450,7 +504,7 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
     }
   }
   // PP (kick.f90:280-298)
-  if ((mask & 8) && cfg.pp) pp_phase(ires_h, in, ppout.p);
+  if ((mask & 8) && cfg.pp) pp_phase(ires_h, cmax_h, in, ppout.p);
   // collect reductions
   PMOut po[4];
   PPOut pq;
@@ -1078,6 +1132,11 @@ int cube_p3m_timing_report(cube_p3m_ctx* ctx, char* buf, int64_t len) {
     char extra[128];
     snprintf(extra, sizeof(extra), "pp_pairs_last %lld 0 0 0\n", (long long)ctx->last_pairs);
     r += extra;
+    for (int l = 2; l <= 5; ++l)
+      if (ctx->pp_unroll[l]) {
+        snprintf(extra, sizeof(extra), "pp_unroll_level%d %d 0 0 0\n", l, ctx->pp_unroll[l]);
+        r += extra;
+      }
     if (buf && len > 0) {
       std::strncpy(buf, r.c_str(), (size_t)len - 1);
       buf[len - 1] = 0;

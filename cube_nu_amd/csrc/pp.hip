@@ -124,8 +124,10 @@ __global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restri
 }
 
 constexpr int PB = 4;       // physical PP cells per brick per dim
-constexpr int kUnroll = 6;  // candidate loads in flight per lane (measured best of 4, 6, 8)
-static_assert(kUnroll <= kPPPad, "entries are padded for kUnroll reads past a run");
+// Candidate loads in flight per lane: U in {4, 6, 8}, chosen per PP level at
+// run time by the engine's autotuner (every U sums the pairs in the same
+// order, so the choice never changes a result).
+static_assert(8 <= kPPPad, "entries are padded for the largest U of reads past a run");
 
 
 // F(r, r_pp) = c Q(ep), ep = 2r/r_pp < 2, with Q(ep) = pi(ep) for ep <= 1 and
@@ -235,9 +237,9 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // above FP32 rounding, so every pair the reference tests in range is tested
 // here with the reference's FP32 operations and the interacting set is the
 // reference's.  The sub-bin starts of the (PB+2)^3 neighbourhood are staged
-// in LDS; candidates are read kUnroll at a time from L1/L2 (the entry array
-// is padded by kPPPad >= kUnroll entries so the reads need no clamp).
-template <typename R>
+// in LDS; candidates are read U at a time from L1/L2 (the entry array is
+// padded by kPPPad >= U entries so the reads need no clamp).
+template <typename R, int U>
 __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
@@ -336,12 +338,12 @@ __global__ void __launch_bounds__(256) k_pp_force(PPLevel L, const PPBox* __rest
         if (gyz) ph0 = ph1 = 0;
         const unsigned phn = (unsigned)(ph1 - ph0);
         const float4* run = ent + beg;
-        for (int p = 0; p < nn; p += kUnroll) {
-          float4 o[kUnroll];
+        for (int p = 0; p < nn; p += U) {
+          float4 o[U];
 #pragma unroll
-          for (int k = 0; k < kUnroll; ++k) o[k] = run[p + k];
+          for (int k = 0; k < U; ++k) o[k] = run[p + k];
 #pragma unroll
-          for (int k = 0; k < kUnroll; ++k) {
+          for (int k = 0; k < U; ++k) {
             const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
             const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
             if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
@@ -434,15 +436,17 @@ void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, cons
 }
 
 void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, float* vp, PPScratch& S, PPOut* out,
-              hipStream_t s) {
+              hipStream_t s, int unroll) {
   const int nbr = (L.npgrid + PB - 1) / PB;
-  const int64_t grid = (int64_t)nbr * nbr * nbr * nbox;
+  const dim3 grid((unsigned)((int64_t)nbr * nbr * nbr * nbox));
   if (precision == 64)
-    hipLaunchKernelGGL(k_pp_force<double>, dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.entries, vp,
-                       out);
+    hipLaunchKernelGGL((k_pp_force<double, 6>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
+  else if (unroll == 4)
+    hipLaunchKernelGGL((k_pp_force<float, 4>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
+  else if (unroll == 8)
+    hipLaunchKernelGGL((k_pp_force<float, 8>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
   else
-    hipLaunchKernelGGL(k_pp_force<float>, dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.entries, vp,
-                       out);
+    hipLaunchKernelGGL((k_pp_force<float, 6>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
   CUBE_HIP(hipGetLastError());
 }
 

@@ -84,6 +84,25 @@ def test_pp_pair_set_is_the_references(gpu, data, tfs):
     assert n_orc > 0 and n_gpu == n_orc
 
 
+def test_pp_autotune_keeps_results(gpu, data):
+    """The first PP call of a level picks the kernel's load depth by timing
+    (4, 6 or 8 candidates in flight); a fresh context tunes again, and the
+    velocities are the same bits whatever depth each picked."""
+    xp, vp, pid, rhoc = data
+    v1 = vp.copy()
+    gpu.kick(xp, v1, rhoc, a_mid=A_MID, dt=DT, phase_mask=8)
+    tuned = {k: v["calls"] for k, v in gpu.timing_report().items() if k.startswith("pp_unroll_level")}
+    print("autotuned", tuned)
+    assert tuned and set(tuned.values()) <= {4, 6, 8}
+    p2 = P3M(NG, izipx=2, precision=32)
+    try:
+        v2 = vp.copy()
+        p2.kick(xp, v2, rhoc, a_mid=A_MID, dt=DT, phase_mask=8)
+    finally:
+        p2.close()
+    assert np.array_equal(v1, v2)
+
+
 def test_pm1_density_and_testrho(gpu, data, tfs):
     xp, vp, pid, rhoc = data
     v = vp.copy()

@@ -411,8 +411,13 @@ __global__ void k_dep_cap(PMLevel L, const PMBox* __restrict__ boxes, int nbox, 
 // same regions as k_deposit's mesh-aligned blocks; each thread deposits one
 // listed particle at a time (taps outside the block skipped), then the block
 // writes its mesh cells.
+// 512 threads (eight wavefronts) share a block's 32 KB accumulator: twice the
+// wavefronts per CU of 256-thread workgroups to cover the LDS atomics' latency
+// (measured 11.9 -> 10.9 ms for the 512^3 PM3 deposit).
+constexpr int kDepListThreads = 512;
+
 template <typename R, int IZ>
-__global__ void __launch_bounds__(256) k_deposit_lists(PMLevel L, const PMBox* __restrict__ boxes,
+__global__ void __launch_bounds__(kDepListThreads) k_deposit_lists(PMLevel L, const PMBox* __restrict__ boxes,
                                                        const int64_t* __restrict__ start,
                                                        const int32_t* __restrict__ used,
                                                        const unsigned long long* __restrict__ ents,
@@ -1222,11 +1227,11 @@ void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes, int nbox,
                       float mass, hipStream_t s) {
   const int64_t grid = (int64_t)nbox * pm_dep_blocks(L);
   if (izipx == 2)
-    hipLaunchKernelGGL((k_deposit_lists<R, 2>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.count,
-                       S.ents, mesh, mass);
+    hipLaunchKernelGGL((k_deposit_lists<R, 2>), dim3((unsigned)grid), dim3(kDepListThreads), 0, s, L, boxes,
+                       S.start, S.count, S.ents, mesh, mass);
   else
-    hipLaunchKernelGGL((k_deposit_lists<R, 1>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start, S.count,
-                       S.ents, mesh, mass);
+    hipLaunchKernelGGL((k_deposit_lists<R, 1>), dim3((unsigned)grid), dim3(kDepListThreads), 0, s, L, boxes,
+                       S.start, S.count, S.ents, mesh, mass);
   CUBE_HIP(hipGetLastError());
 }
 template void pm_deposit_lists<float>(int, const PMLevel&, const PMBox*, int, const DepBins&, float*, float,

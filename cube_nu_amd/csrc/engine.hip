@@ -393,9 +393,15 @@ int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
   static const int cand[3] = {4, 6, 8};
   vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
   otune.ensure(1);
-  hipEvent_t e0, e1;
-  CUBE_HIP(hipEventCreate(&e0));
-  CUBE_HIP(hipEventCreate(&e1));
+  struct Ev {  // destroyed on every exit, errors included
+    hipEvent_t e = nullptr;
+    ~Ev() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } a, b;
+  CUBE_HIP(hipEventCreate(&a.e));
+  CUBE_HIP(hipEventCreate(&b.e));
+  hipEvent_t e0 = a.e, e1 = b.e;
   float best = 0;
   int pick = 6;
   for (int rep = 0; rep < 2; ++rep)
@@ -413,8 +419,6 @@ int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
         pick = u;
       }
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   pp_unroll[iapm] = pick;
   return pick;
 }

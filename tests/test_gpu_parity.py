@@ -71,6 +71,32 @@ def test_phase_parity(gpu, data, tfs, mask):
         assert out_g["napm"] == out_o["napm"]
 
 
+@pytest.fixture(scope="module")
+def data_fine():
+    # 8000 and 21000 particles in one coarse cell: npc_max 125 and 328 put
+    # their sub-tiles at ires = 4 (ratio_sf 6: 24 mesh cells per coarse cell,
+    # the divided, non-power-of-two mesh scale) and ires = 5 (ratio_sf 8)
+    dense = [((5, 9, 2), 8000), ((10, 3, 11), 21000)]
+    return buffered_set(_geo(), seed=23, mean=6.0, dense=dense)
+
+
+@pytest.mark.parametrize("mask", [4, 8, 0])
+def test_fine_pm3_levels_parity(gpu, data_fine, tfs, mask):
+    """PM3 at ires = 4 and 5, PP in their sub-tiles, and the full kick."""
+    xp, vp, pid, rhoc = data_fine
+    v_gpu = vp.copy()
+    v_orc = vp.copy()
+    out_g = gpu.kick(xp, v_gpu, rhoc, a_mid=A_MID, dt=DT, phase_mask=mask)
+    out_o, _, _ = orc.kick(oparams(_geo()), xp, v_orc, rhoc, tfs, A_MID, DT, phase_mask=mask)
+    assert out_o["napm"][3] == 1 and out_o["napm"][4] == 1
+    assert out_g["napm"] == out_o["napm"]
+    err = rel_rms(v_gpu - vp, v_orc - vp)
+    print(f"mask {mask}: rms rel dv err {err:.3e}")
+    assert err <= 1e-5
+    for k in {4: ["f2max_pm3"], 8: ["f2max_pp"], 0: ["dt_pm3", "dt_pp", "dt_vmax", "vz_max"]}[mask]:
+        assert abs(out_g[k] - out_o[k]) <= 1e-5 * abs(out_o[k]), k
+
+
 def test_pp_pair_set_is_the_references(gpu, data, tfs):
     """The PP kernel tests only the x window and the y/z columns that can hold
     partners; the number of interacting pairs (physical-physical counted once,

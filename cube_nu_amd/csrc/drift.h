@@ -28,8 +28,13 @@ struct DriftScratch {
 };
 
 // Error bits written to flags[0].
-enum { DRIFT_OUT_OF_RANGE = 1, DRIFT_PID_ZERO = 2, DRIFT_SHIFT_BOUND = 8 };
+enum { DRIFT_OUT_OF_RANGE = 1, DRIFT_PID_ZERO = 2, DRIFT_SHIFT_BOUND = 8, DRIFT_TILE_OVERFLOW = 16 };
 
+// Tile overflow check (drift.f90:85-93) on the device: flags[4] = the largest
+// extended tile count (off[(T+1) nct] - off[T nct]), DRIFT_TILE_OVERFLOW in
+// flags[0] when one exceeds np_tile_max.
+void drift_tile_check(const int64_t* off, int64_t ncell_tile, int ntiles, int64_t np_tile_max, int32_t* flags,
+                      hipStream_t s);
 // Pass 1: destination range check and maximum shift.
 void drift_shift(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
                  int64_t ncells, DriftScratch& S, hipStream_t s);

@@ -286,6 +286,15 @@ __global__ void k_drift_rhoc(DriftParams P, int64_t ncells, const int32_t* __res
   rhoc[c] = physical(P, ijk) ? newcount[pcell(P, T, ijk)] : 0;
 }
 
+__global__ void k_tile_check(const int64_t* __restrict__ off, int64_t nct, int ntiles, int64_t cap,
+                             int32_t* __restrict__ flags) {
+  for (int T = threadIdx.x; T < ntiles; T += blockDim.x) {
+    const int64_t n = off[(int64_t)(T + 1) * nct] - off[(int64_t)T * nct];
+    atomic_max_int(&flags[4], (int32_t)(n < (int64_t)INT32_MAX ? n : (int64_t)INT32_MAX));
+    if (n > cap) atomicOr(&flags[0], DRIFT_TILE_OVERFLOW);
+  }
+}
+
 __global__ void k_buffer_rhoc(Geo g, const int32_t* __restrict__ rp, int32_t* __restrict__ re,
                               int32_t* __restrict__ physcount) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -309,6 +318,12 @@ __global__ void k_buffer_rhoc(Geo g, const int32_t* __restrict__ rp, int32_t* __
 
 
 }  // namespace
+
+void drift_tile_check(const int64_t* off, int64_t ncell_tile, int ntiles, int64_t np_tile_max, int32_t* flags,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_check, dim3(1), dim3(256), 0, s, off, ncell_tile, ntiles, np_tile_max, flags);
+  CUBE_HIP(hipGetLastError());
+}
 
 void drift_shift(const DriftParams& P, const void* xp, const float* vp, const int32_t* rhoc, const int64_t* off,
                  int64_t ncells, DriftScratch& S, hipStream_t s) {

@@ -22,6 +22,10 @@ using namespace cube;
 
 namespace {
 
+// Device buffer that only grows.  A growth allocates 1/16 more than asked:
+// the particle sets, the PP entries and the deposit lists creep up by a few
+// elements from step to step, and an exact-size regrowth would free and
+// reallocate gigabytes (about a millisecond of an idle GPU) each time.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -29,16 +33,15 @@ struct DevBuf {
   T* ensure(size_t m) {
     if (m > n) {
       release();
-      if (m) {
-        hipError_t e = hipMalloc(&p, m * sizeof(T));
-        if (e != hipSuccess) {
-          p = nullptr;
-          n = 0;
-          throw Error(CUBE_ERR_HIP, "hipMalloc of " + std::to_string(m * sizeof(T)) + " bytes failed: " +
-                                        hipGetErrorString(e));
-        }
+      const size_t cap = m + m / 16;
+      hipError_t e = hipMalloc(&p, cap * sizeof(T));
+      if (e != hipSuccess) {
+        p = nullptr;
+        n = 0;
+        throw Error(CUBE_ERR_HIP, "hipMalloc of " + std::to_string(cap * sizeof(T)) + " bytes failed: " +
+                                      hipGetErrorString(e));
       }
-      n = m;
+      n = cap;
     }
     return p;
   }
@@ -145,10 +148,11 @@ struct cube_p3m_ctx {
   int64_t capb[2] = {0, 0};
   void ensure_set(int b, int64_t c) {
     if (c > capb[b]) {
-      xp[b].ensure((size_t)c * 3 * esz());
-      vp[b].ensure((size_t)c * 3);
-      pid[b].ensure((size_t)c);
-      capb[b] = c;
+      const int64_t cc = c + c / 16;  // room for the next steps' growth
+      xp[b].ensure((size_t)cc * 3 * esz());
+      vp[b].ensure((size_t)cc * 3);
+      pid[b].ensure((size_t)cc);
+      capb[b] = cc;
     }
     rhoc[0].ensure(g.ncells_ext);
     rhoc[1].ensure(g.ncells_ext);
@@ -569,29 +573,23 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
   P.nt = (int)g.nt; P.nte = (int)g.nte; P.ncb = (int)g.ncb; P.nnt = (int)g.nnt; P.rcs = (int)g.rcs;
   P.izipx = (int)g.izipx;
   out.nlayer = P.nlayer;
-  // tile overflow (drift.f90:85-93): every particle of the extended tile lands
-  // in the tile's rhoce range, so its total is the extended tile count.
-  std::vector<int64_t> toff(g.ntiles + 1);
-  for (int64_t T = 0; T <= g.ntiles; ++T)
-    CUBE_HIP(hipMemcpyAsync(&toff[T], off.p + T * g.ncell_tile, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CUBE_HIP(hipStreamSynchronize(s));
-  float overhead = 0;
-  for (int64_t T = 0; T < g.ntiles; ++T) {
-    int64_t n = toff[T + 1] - toff[T];
-    overhead = std::max(overhead, (float)n / (float)g.np_tile_max);
-    if (n > g.np_tile_max)
-      throw Error(CUBE_ERR_TILE_OVERFLOW, "too many particles in this tile+buffer: " + std::to_string(n) + " > " +
-                                              std::to_string(g.np_tile_max) + "; please set tile_buffer larger");
-  }
-  out.overhead_tile = overhead;
   const int64_t nt3 = g.nt * g.nt * g.nt, nphys_cells = g.ntiles * nt3;
   DriftScratch S;
   S.flags = dflags.p;
   S.newcount = dnewcount.ensure(nphys_cells);
   S.base = dbase.ensure(nphys_cells + 1);
   const int other = 1 - cur;
-  int32_t fl[4];
+  int32_t fl[5];
   int64_t nplocal = 0;
+  // tile overflow (drift.f90:85-93): every particle of the extended tile lands
+  // in the tile's rhoce range, so its total is the extended tile count; the
+  // check runs on the device and is read back with the pass's flags (the
+  // passes before the read write scratch arrays only)
+  auto start_pass = [&]() {
+    zero_async(S.flags, sizeof(int32_t) * 8, s);
+    zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
+    drift_tile_check(off.p, g.ncell_tile, (int)g.ntiles, g.np_tile_max, S.flags, s);
+  };
   auto table_tiles = [&](int64_t Dt) {  // tiles per rank/offset batch within a 2 GiB table
     const int64_t per_tile = g.ncell_tile * Dt;
     const int64_t tbudget = std::max<int64_t>(per_tile, ((int64_t)2 << 30) / 4);
@@ -605,6 +603,10 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
     }
     CUBE_HIP(hipMemcpyAsync(&nplocal, S.base + nphys_cells, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipStreamSynchronize(s));
+    if (fl[0] & DRIFT_TILE_OVERFLOW)
+      throw Error(CUBE_ERR_TILE_OVERFLOW, "too many particles in this tile+buffer: " + std::to_string(fl[4]) + " > " +
+                                              std::to_string(g.np_tile_max) + "; please set tile_buffer larger");
+    out.overhead_tile = (float)fl[4] / (float)g.np_tile_max;
     if (fl[0] & DRIFT_OUT_OF_RANGE)
       throw Error(CUBE_ERR_OUT_OF_RANGE, "error: particle out of range (drift beyond 2*ncb coarse cells)");
   };
@@ -641,8 +643,7 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
     }
     const int64_t Dt = (int64_t)P.D[0] * P.D[1] * P.D[2];
     S.table = dtable.ensure((size_t)(g.ntiles * g.ncell_tile * Dt));
-    zero_async(S.flags, sizeof(int32_t) * 8, s);
-    zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
+    start_pass();
     zero_async(S.table, sizeof(int32_t) * (size_t)(g.ntiles * g.ncell_tile * Dt), s);
     {
       KScope k(&tm, "drift_rank", (double)np * (3 * esz() + 16));
@@ -656,8 +657,7 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
     }
   }
   if (!fused) {
-    zero_async(S.flags, sizeof(int32_t) * 8, s);
-    zero_async(S.newcount, sizeof(int32_t) * nphys_cells, s);
+    start_pass();
     {
       KScope k(&tm, "drift_shift", (double)np * (3 * esz() + 12));
       drift_shift(P, xpc(), vpc(), rhocc(), off.p, g.ncells_ext, S, s);

@@ -159,6 +159,7 @@ def test_drift_bit_exact(gpu, data):
     og = gpu.drift(xg, vg, pg, rg, dt_old=0.4, dt=0.6, vz_max=vz)
     n = og["nplocal"]
     assert n == oo["nplocal"] and og["nlayer"] == oo["nlayer"]
+    assert og["overhead_tile"] == oo["overhead_tile"]
     assert np.array_equal(rg, ro)
     assert np.array_equal(xg[:n], xo)
     assert np.array_equal(vg[:n], vo)
@@ -252,6 +253,22 @@ def test_resident_drift_after_kick_bit_exact():
         assert st == 0 and d["nplocal"] == oo["nplocal"] and d["nlayer"] == oo["nlayer"]
         assert np.array_equal(rg, ro)
         assert np.array_equal(xg, xo) and np.array_equal(vg, vo) and np.array_equal(pg, po)
+    finally:
+        p.close()
+
+
+def test_drift_tile_overflow_error(data):
+    """np_tile_max (tile_buffer) smaller than an extended tile's particle count
+    stops the drift with the tile overflow status (drift.f90:87-93) before any
+    particle moves."""
+    from cube_nu_amd.p3m import CUBE_ERR_TILE_OVERFLOW
+
+    xp, vp, pid, rhoc = data
+    p = P3M(NG, izipx=2, precision=32, tile_buffer=0.05)
+    try:
+        with pytest.raises(CubeError) as e:
+            p.drift(xp.copy(), vp.copy(), pid.copy(), rhoc.copy(), dt_old=0.4, dt=0.6, vz_max=1.0)
+        assert e.value.code == CUBE_ERR_TILE_OVERFLOW
     finally:
         p.close()
 

@@ -485,7 +485,11 @@ __global__ void __launch_bounds__(128) k_pm1_stencil(PMLevel L, const XT<IZ>* __
                                                      const int32_t* __restrict__ rhoc,
                                                      const int64_t* __restrict__ off, void* __restrict__ stencil) {
   using Acc = typename std::conditional<std::is_same<R, float>::value, unsigned long long, double>::type;
-  __shared__ Acc red[2][27][65];
+  // the 27 tap sums leave through LDS in two rounds of at most 14 taps: half
+  // the LDS of one round, so the register limit (five wavefronts per SIMD)
+  // rather than LDS sets the occupancy
+  constexpr int kRound = 14;
+  __shared__ Acc red[2][kRound][65];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t ncell = (int64_t)L.n * L.n * L.n;
   const int64_t g = (int64_t)blockIdx.x * 2 + wave;
@@ -511,14 +515,22 @@ __global__ void __launch_bounds__(128) k_pm1_stencil(PMLevel L, const XT<IZ>* __
     }
   }
 #pragma unroll
-  for (int q = 0; q < 27; ++q) red[wave][q][lane] = a27[q];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane < 27) {
-    Acc s = 0;
-    for (int l = 0; l < 64; ++l) s += red[wave][lane][l];  // fixed order
-    ((Acc*)stencil)[(int64_t)lane * ncell + g] = s;
+  for (int h = 0; h < 2; ++h) {
+    const int q0 = h * kRound, nq = h ? 27 - kRound : kRound;
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+      if (q >= q0 && q < q0 + nq) red[wave][q - q0][lane] = a27[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < nq) {
+      Acc s = 0;
+      for (int l = 0; l < 64; ++l) s += red[wave][lane][l];  // fixed order
+      ((Acc*)stencil)[(int64_t)(q0 + lane) * ncell + g] = s;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 

@@ -239,16 +239,17 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // reference's.  The sub-bin starts of the (PB+2)^3 neighbourhood are staged
 // in LDS; candidates are read U at a time from L1/L2 (the entry array is
 // padded by kPPPad >= U entries so the reads need no clamp).
-// FP32 form: at most 64 VGPRs, so eight wavefronts fit per SIMD to cover the
+// FP32 form: eight 256-thread workgroups per CU (the launch bound's second
+// value), i.e. at most 64 VGPRs and eight wavefronts per SIMD to cover the
 // candidate loads' latency (28.5 vs 30.2 ms at 512^3 with 71 VGPRs and seven
 // wavefronts, a few values spilled); the FP64 form keeps its registers.
 template <typename R>
-constexpr int pp_waves_per_eu() {
+constexpr int pp_min_blocks_per_cu() {
   return sizeof(R) == 4 ? 8 : 1;
 }
 
 template <typename R, int U>
-__global__ void __launch_bounds__(256, pp_waves_per_eu<R>()) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
+__global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
                                                   PPOut* __restrict__ out) {

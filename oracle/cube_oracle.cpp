@@ -16,7 +16,9 @@
 // Index conventions: arrays are the reference's column-major arrays passed as
 // flat C pointers; particle indices are 0-based here (Fortran ip-1).
 //
-// Parity status per function is recorded in the header table of DESIGN.md.
+// Every function is checked against an independent statement in
+// tests/test_oracle.py (closed forms, numpy restatements, brute-force sums,
+// invariants) before it is trusted as the checker of the GPU path.
 #include <algorithm>
 #include <cmath>
 #include <complex>

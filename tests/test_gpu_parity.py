@@ -2,7 +2,7 @@
 identical seeded inputs (ng=64: 262k particles, 64 tiles of 4^3 coarse cells,
 4096 sub-tiles).
 
-Tolerances (north star): rms relative error of the velocity change (i.e. of
+Tolerances: rms relative error of the velocity change (i.e. of
 the force) <= 1e-5 per phase; integer outputs (napm, drift order, counts,
 re-encoded coordinates, PID permutation) bit-exact.
 """

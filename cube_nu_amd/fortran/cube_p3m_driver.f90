@@ -10,7 +10,9 @@
 ! reference's checkpoint streams, checkpoint.f90:11-33): xp.bin (int16,
 ! 3*np), vp.bin (real32, 3*np), pid.bin (int64, np), rhoc.bin (int32), then
 !   1. kick (cube_p3m_kick) and writes vp_kick.bin and kick.txt,
-!   2. drift with the kicked velocities (cube_p3m_drift) and writes
+!   2. buffer_v (cube_p3m_buffer_v): the buffer copies take the kicked
+!      velocities, as main.f90:37-38 orders kick and buffer_v,
+!   3. drift with the kicked velocities (cube_p3m_drift) and writes
 !      xp_drift.bin, vp_drift.bin, pid_drift.bin, rhoc_drift.bin,
 !      spine.bin (idx_b_l, idx_b_r, ppl0, pplr, pprl, ppr0, ppl, ppr) and
 !      drift.txt.
@@ -103,6 +105,7 @@ program cube_p3m_driver
   write(u, '(7i12)') kout%napm
   write(u, '(es24.16)') kout%testrho
   close(u)
+  call cube_check(cube_p3m_buffer_v(ctx, vp, pid, rhoc, np), ctx, 'buffer_v')
 
   din%dt_old = dt_old; din%dt = dt_drift; din%vz_max = kout%vz_max
   call cube_check(cube_p3m_drift(ctx, xp, vp, pid, rhoc, np, idx_b_l, idx_b_r, sp(:, 1), sp(:, 2), sp(:, 3), &

@@ -40,9 +40,12 @@ def test_fortran_host_kick_and_drift(tmp_path):
     for k, v in zip(("dt_pm1", "dt_pm2", "dt_pm3", "dt_pp", "dt_vmax", "vz_max"), dts):
         assert abs(v - out_o[k]) <= 1e-5 * abs(out_o[k]), k
     assert [int(v) for v in txt[6:13]] == out_o["napm"]
-    # drift of the GPU-kicked velocities, bit-exact against the oracle
+    # buffer_v, then the drift of the GPU-kicked velocities, bit-exact against
+    # the oracle
     vz = np.float32(dts[5])
-    st, xe, ve, pe, re, oo = orc.drift(P, xp, vk, pid, rhoc, dt_old, dt_drift, float(vz))
+    vkb = vk.copy()
+    orc.buffer_v(P, vkb, rhoc)
+    st, xe, ve, pe, re, oo = orc.drift(P, xp, vkb, pid, rhoc, dt_old, dt_drift, float(vz))
     assert st == 0
     m = oo["nplocal"]
     nplocal, _, nlayer = open(tmp_path / "drift.txt").read().split()

@@ -106,3 +106,35 @@ def test_pm2_with_80_point_tiles_matches_oracle():
         assert abs(out_g["f2max_pm2"] - out_o["f2max_pm2"]) <= 1e-5 * abs(out_o["f2max_pm2"])
     finally:
         p.close()
+
+
+def test_other_tile_decomposition_matches_oracle():
+    """nnt = nns = 2 (2^3 tiles of 8^3 coarse cells, 2^3 sub-tiles each): other
+    tile, sub-tile, mesh (64^3 PM2, 48^3 PM3) and PP grid (8^3) sizes than the
+    default decomposition; full kick within the tolerance, drift bit-exact."""
+    geo = Geometry(64, izipx=2, nnt=2, nns=2)
+    data = buffered_set(geo, seed=29, mean=6.0, dense=[((3, 7, 12), 2000)])
+    P = oparams(geo)
+    tfs = orc.tf_default(P)
+    p = P3M(64, izipx=2, precision=32, nnt=2, nns=2)
+    try:
+        xp, vp, pid, rhoc = data
+        v_gpu = vp.copy()
+        v_orc = vp.copy()
+        out_g = p.kick(xp, v_gpu, rhoc, a_mid=A_MID, dt=DT)
+        out_o, _, _ = orc.kick(P, xp, v_orc, rhoc, tfs, A_MID, DT)
+        err = rel_rms(v_gpu - vp, v_orc - vp)
+        print(f"nnt = nns = 2: rms rel dv err {err:.3e}, napm {out_g['napm']}")
+        assert err <= 1e-5 and out_g["napm"] == out_o["napm"]
+        for k in ("dt_pm1", "dt_pm2", "dt_pm3", "dt_pp", "dt_vmax", "vz_max"):
+            assert abs(out_g[k] - out_o[k]) <= 1e-5 * abs(out_o[k]), k
+        vz = float(np.abs(vp[:, 2]).max())
+        st, xo, vo, po, ro, oo = orc.drift(P, xp, vp, pid, rhoc, 0.4, 0.6, vz)
+        assert st == 0
+        xg, vg, pg, rg = xp.copy(), vp.copy(), pid.copy(), rhoc.copy()
+        og = p.drift(xg, vg, pg, rg, dt_old=0.4, dt=0.6, vz_max=vz)
+        n = og["nplocal"]
+        assert n == oo["nplocal"] and np.array_equal(rg, ro)
+        assert np.array_equal(xg[:n], xo) and np.array_equal(vg[:n], vo) and np.array_equal(pg[:n], po)
+    finally:
+        p.close()

@@ -4,8 +4,8 @@
 Workload (default): 512^3 CDM particles in a 600 Mpc/h box, seeded Zel'dovich
 displacements at z=2 (BBKS P(k), h=0.6817, Om=0.301, Ob=0.048, ns=0.9693,
 sigma8=0.8), int16 compressed coordinates.  One step = drift (re-sort and
-re-encode) + nn=1 buffer layout + kick (PM1 + PM2 + PM3 + PP), the order of
-main.f90:34-38.  Inputs are generated on the GPU and are resident in HBM when
+re-encode) + nn=1 buffer layout + kick (PM1 + PM2 + PM3 + PP) + buffer_v, the
+order of main.f90:34-38.  Inputs are generated on the GPU and are resident in HBM when
 the timed region starts.  value = particles * steps / seconds.
 
 Prints ONE JSON line (see the driver contract) with a `roofline` object for

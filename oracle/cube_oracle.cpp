@@ -509,6 +509,20 @@ struct LevelBox {
 
 void pm_box(const Geo& g, const void* xp, float* vp, const int32_t* rhoc, const std::vector<int64_t>& off,
             const float* tf, const LevelBox& b, const OKickIn& kin, float* f2max_t, float vmax_t[3]) {
+  // A box whose deposit window holds no particle has a zero density, hence a
+  // zero potential and force (f2max_t = 0), and no physical particle to kick:
+  // its result is known without the transforms.
+  {
+    int64_t npart = 0;
+    for (int64_t k = b.nc1[2] - b.nloop; k <= b.nc2[2] + b.nloop && npart == 0; ++k)
+      for (int64_t j = b.nc1[1] - b.nloop; j <= b.nc2[1] + b.nloop; ++j)
+        for (int64_t i = b.nc1[0] - b.nloop; i <= b.nc2[0] + b.nloop; ++i) npart += rhoc[g.rc(i, j, k, b.T)];
+    if (npart == 0) {
+      *f2max_t = 0;
+      for (int d = 0; d < 3; ++d) vmax_t[d] = 0;
+      return;
+    }
+  }
   const int64_t L = b.nphy + 2 * b.nex;
   std::vector<float> rth((size_t)L * L * L, 0.0f);
   auto R = [&](int64_t i, int64_t j, int64_t k) -> float& {

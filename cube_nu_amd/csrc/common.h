@@ -130,16 +130,13 @@ struct FDWeights {
   }
 };
 
-// Kick update (particle_mesh.f90:205, kick.f90:797):
-// vp = vp + dv*a_mid*dt/6/pi, ((dv*a_mid)*dt)/6 in real(4), /pi and + in FP64.
-__device__ inline float kick_update(float vp, float dv, float a_mid, float dt) {
-  float t = ((dv * a_mid) * dt) / 6.0f;
-  return (float)((double)vp + (double)t / kPi);
-}
-__device__ inline double kick_update(double vp, double dv, float a_mid, float dt) {
-  double t = ((dv * (double)a_mid) * (double)dt) / 6.0;
-  return vp + t / kPi;
-}
+// Kick update (particle_mesh.f90:205, kick.f90:797): vp = vp + dv*a_mid*dt/6/pi.
+// The coefficient kc = a_mid*dt/(6 pi) is formed on the host in FP64 (the
+// reference rounds a_mid*dt to real(4) first; here the kick and drift
+// coefficients stay FP64), and the product and add run in FP64 before the
+// store to the FP32 velocity.
+__device__ inline float kick_update(float vp, float dv, double kc) { return (float)((double)vp + (double)dv * kc); }
+__device__ inline double kick_update(double vp, double dv, double kc) { return vp + dv * kc; }
 
 // F_ra_inline (kick.f90:360-374) with integer powers expanded as products.
 // c must equal (1./35.)/apm**2 evaluated in R (f_ra_coef), passed in so that

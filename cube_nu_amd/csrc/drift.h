@@ -8,7 +8,7 @@ namespace cube {
 constexpr int kMaxShift = 16;  // largest cell shift the offset pass orders
 
 struct DriftParams {
-  float dt_mid;
+  double dt_mid;  // (dt_old + dt)/2 in FP64 (the reference's real(4) coefficient, kept FP64 here)
   int nlayer;
   int nt, nte, ncb, nnt, rcs, izipx;
   int M[3];       // maximum |cell shift| per axis

@@ -36,8 +36,8 @@ template <int IZ>
 __device__ inline void dest_cell(const DriftParams& P, const XT<IZ>* xp, const float* vp, int64_t ip,
                                  const int ijk[3], int g[3]) {
   for (int d = 0; d < 3; ++d) {
-    const float wv = P.dt_mid * vp[3 * ip + d];
-    const double dx = (double)(wv / (float)P.rcs);               // drift.f90:57
+    const double wv = P.dt_mid * (double)vp[3 * ip + d];        // FP64 coefficient
+    const double dx = wv / (double)P.rcs;                        // drift.f90:57
     const double xq = ((double)ijk[d] - 1.0) + XP<IZ>::off(xp, ip, d);  // drift.f90:59
     g[d] = (int)ceil(xq + dx);                                   // drift.f90:63
   }
@@ -263,8 +263,8 @@ __global__ void k_drift_scatter(DriftParams P, const XT<IZ>* __restrict__ xp, co
     dest_cell<IZ>(P, xp, vp, ip, ijk, g);
     const int64_t pos = base_out[pcell(P, T, g)] + table[cl * Dt + dcode(P, g, ijk)] + r;
     for (int d = 0; d < 3; ++d) {
-      const float wv = P.dt_mid * vp[3 * ip + d];
-      const double sh = (double)wv / (xr * (double)P.rcs);
+      const double wv = P.dt_mid * (double)vp[3 * ip + d];
+      const double sh = wv / (xr * (double)P.rcs);
       const long long dn = (long long)round(sh);  // nint: half away from zero
       const long long nv = (long long)xp[3 * ip + d] + dn;
       if (IZ == 2) xo[3 * pos + d] = (int16_t)(uint16_t)(nv & 0xffff);

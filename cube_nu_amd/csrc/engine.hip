@@ -60,6 +60,10 @@ inline float g_grid() { return (float)((double)(1.0f / 6.0f) / kPi); }
 inline float dt_from_f2(float c, float f2, float a_mid) {
   return c * std::sqrt(1.0f / ((std::sqrt(f2) * a_mid) * g_grid()));
 }
+// Kick coefficient a_mid*dt/6/pi (particle_mesh.f90:205, kick.f90:574, 797)
+// in FP64: the reference rounds a_mid*dt to real(4); here the kick and drift
+// coefficients stay FP64 on the GPU.
+inline double kick_coef(const cube_kick_in& in) { return ((in.a_mid * in.dt) / 6.0) / kPi; }
 
 }  // namespace
 
@@ -291,7 +295,7 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
     else fft_convolve<R>(fc, mesh, cplx, G, nmesh, n3, (R)0, s, &tm, tag);                // kick.f90:846
     {
       KScope k(&tm, (t + "interp_kick").c_str(), cells * sizeof(R));
-      pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, in.a_mid, in.dt, o, s);
+      pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, kick_coef(in), o, s);
     }
   }
 }
@@ -304,7 +308,7 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
     mass_scale = (float)((double)in.mass_p_cdm * ((1.0 - cfg.f_nu) * (1.0 - cfg.f_nu)));
   else
     mass_scale = in.mass_p_cdm;
-  const float v_scale = (float)((double)((in.a_mid * in.dt) / 6.0f) / kPi);
+  const double v_scale = kick_coef(in);
   const int64_t nns = g.nns;
   // sub-tiles by descending occupancy (kick.f90:132-138 sorts them by npc_max
   // for schedule(dynamic,1)): the workgroups of the densest sub-tiles are
@@ -527,7 +531,7 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
   float vmax[3] = {0, 0, 0};
   if (mask & 1) {
     out.f2max_pm1 = f2(po[0]);
-    out.dt_pm1 = dt_from_f2(0.5f, out.f2max_pm1, in.a_mid);
+    out.dt_pm1 = dt_from_f2(0.5f, out.f2max_pm1, (float)in.a_mid);
     unsigned long long tfx = 0;
     double td = 0;
     for (int q = 0; q < kSumSlots; ++q) {
@@ -539,17 +543,17 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
   }
   if (mask & 2) {
     out.f2max_pm2 = f2(po[1]);
-    out.dt_pm2 = dt_from_f2(0.5f, out.f2max_pm2, in.a_mid);
+    out.dt_pm2 = dt_from_f2(0.5f, out.f2max_pm2, (float)in.a_mid);
     for (int d = 0; d < 3; ++d) vmax[d] = vm(po[1], d);
   }
   if ((mask & 4) && cfg.pm3) {
     out.f2max_pm3 = f2(po[2]);
-    out.dt_pm3 = dt_from_f2(0.5f, out.f2max_pm3, in.a_mid);
+    out.dt_pm3 = dt_from_f2(0.5f, out.f2max_pm3, (float)in.a_mid);
     for (int d = 0; d < 3; ++d) vmax[d] = vm(po[2], d);
   }
   if ((mask & 8) && cfg.pp) {
     out.f2max_pp = pq.f2max;
-    out.dt_pp = dt_from_f2(0.05f, pq.f2max, in.a_mid);
+    out.dt_pp = dt_from_f2(0.05f, pq.f2max, (float)in.a_mid);
     for (int d = 0; d < 3; ++d) vmax[d] = pq.vmax[d];
     unsigned long long npp = 0, npg = 0;
     for (int q = 0; q < kSumSlots; ++q) {
@@ -571,8 +575,8 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
   if (layout != 1) throw Error(CUBE_ERR_STATE, "drift needs the buffered (extended) particle layout");
   ensure_offsets();
   DriftParams P{};
-  P.dt_mid = (in.dt_old + in.dt) / 2.0f;                                            // drift.f90:21
-  P.nlayer = 2 * (int)std::ceil((P.dt_mid * in.vz_max) / (float)g.rcs) + 2;         // drift.f90:26
+  P.dt_mid = (in.dt_old + in.dt) / 2.0;                                                     // drift.f90:21
+  P.nlayer = 2 * (int)std::ceil((P.dt_mid * (double)in.vz_max) / (double)g.rcs) + 2;        // drift.f90:26
   P.nt = (int)g.nt; P.nte = (int)g.nte; P.ncb = (int)g.ncb; P.nnt = (int)g.nnt; P.rcs = (int)g.rcs;
   P.izipx = (int)g.izipx;
   out.nlayer = P.nlayer;
@@ -634,8 +638,8 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
   bool fused = vmax_valid;
   int Mb[3] = {0, 0, 0};
   for (int d = 0; d < 3 && fused; ++d) {
-    const float dm = std::fabs((P.dt_mid * last_vmax[d]) / (float)g.rcs);
-    if (!(dm <= (float)kMaxShift)) fused = false;
+    const double dm = std::fabs((P.dt_mid * (double)last_vmax[d]) / (double)g.rcs);
+    if (!(dm <= (double)kMaxShift)) fused = false;
     else Mb[d] = (int)std::ceil(dm);
   }
   if (fused) fused = table_tiles((int64_t)(2 * Mb[0] + 1) * (2 * Mb[1] + 1) * (2 * Mb[2] + 1)) == (int)g.ntiles;

@@ -58,7 +58,7 @@ void pm_level_scale(PMLevel& L);
 
 template <typename R>
 void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp, float* vp,
-               const int32_t* rhoc, const int64_t* cell_off, const R* phi, float a_mid, float dt, PMOut* out,
+               const int32_t* rhoc, const int64_t* cell_off, const R* phi, double kc, PMOut* out,
                hipStream_t s);
 
 // Green_3D (Green/Green.f90:1-80) evaluated in FP64 on the device, stored as

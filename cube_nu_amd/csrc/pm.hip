@@ -590,7 +590,7 @@ template <typename R, int IZ, int BD, int MAXC>
 __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restrict__ boxes,
                                                 const XT<IZ>* __restrict__ xp, float* __restrict__ vp,
                                                 const int32_t* __restrict__ rhoc, const int64_t* __restrict__ off,
-                                                const R* __restrict__ phi, float a_mid, float dt,
+                                                const R* __restrict__ phi, double kc,
                                                 PMOut* __restrict__ out) {
   constexpr int FS = BD + 2, PS = BD + 6;
   constexpr int NF = (FS * FS * FS + 255) / 256;
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
               dv[d] = dv[d] + ((lds[d * FS * FS * FS + li] * w[0][i1]) * w[1][i2]) * w[2][i3];
           }
       for (int d = 0; d < 3; ++d) {
-        const float v = (float)kick_update((R)v0[d], dv[d], a_mid, dt);
+        const float v = (float)kick_update((R)v0[d], dv[d], kc);
         vp[3 * ip + d] = v;
         vm[d] = max(vm[d], (R)fabsf(v));
       }
@@ -799,7 +799,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
                                                       const XT<IZ>* __restrict__ xp, float* __restrict__ vp,
                                                       const int32_t* __restrict__ rhoc,
                                                       const int64_t* __restrict__ off, const R* __restrict__ phi,
-                                                      float a_mid, float dt, PMOut* __restrict__ out) {
+                                                      double kc, PMOut* __restrict__ out) {
   constexpr int FX = IBX + 2, FY = IBY + 2, FZ = IBZ + 2, FN = FX * FY * FZ;
   constexpr int PX = IBX + 6, PY = IBY + 6, PZ = IBZ + 6, PN = PX * PY * PZ;
   constexpr int NF = (FN + IT - 1) / IT, NPH = (PN + IT - 1) / IT;
@@ -946,7 +946,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
             dv[d] = dv[d] + ((lds[d * FN + li] * wt[0][i1]) * wt[1][i2]) * wt[2][i3];
         }
     for (int d = 0; d < 3; ++d) {
-      const float v = (float)kick_update((R)v0[d], dv[d], a_mid, dt);
+      const float v = (float)kick_update((R)v0[d], dv[d], kc);
       vp[3 * ip + d] = v;
       vm[d] = max(vm[d], (R)fabsf(v));
     }
@@ -1146,33 +1146,33 @@ void launch_deposit(const PMLevel& L, const PMBox* boxes, int nbox, const void* 
 
 template <typename R, int IZ, int BD>
 void launch_interp_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
-                      const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
+                      const int64_t* off, const R* phi, double kc, PMOut* out, hipStream_t s) {
   const int nblk = (L.nforce + BD - 1) / BD;
   const int64_t grid = (int64_t)nblk * nblk * nblk * nbox;
   // a block reads at most (BD/w + 1)^3 coarse cells
   if (L.w >= 4)
     hipLaunchKernelGGL((k_interp<R, IZ, BD, (BD / 4 + 1) * (BD / 4 + 1) * (BD / 4 + 1)>), dim3((unsigned)grid),
-                       dim3(256), 0, s, L, boxes, (const XT<IZ>*)xp, vp, rhoc, off, phi, a_mid, dt, out);
+                       dim3(256), 0, s, L, boxes, (const XT<IZ>*)xp, vp, rhoc, off, phi, kc, out);
   else
     hipLaunchKernelGGL((k_interp<R, IZ, BD, (BD + 1) * (BD + 1) * (BD + 1)>), dim3((unsigned)grid), dim3(256), 0,
-                       s, L, boxes, (const XT<IZ>*)xp, vp, rhoc, off, phi, a_mid, dt, out);
+                       s, L, boxes, (const XT<IZ>*)xp, vp, rhoc, off, phi, kc, out);
   CUBE_HIP(hipGetLastError());
 }
 
 template <typename R, int IZ>
 void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
-                   const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out, hipStream_t s) {
+                   const int64_t* off, const R* phi, double kc, PMOut* out, hipStream_t s) {
   if (!L.periodic && !L.xdiv && (L.w == 4 || L.w == 8) && L.nforce % L.w == 0) {
     const int ncell = L.nforce / L.w;
     int64_t grid = nbox;
     const int g[3] = {IBX / L.w, IBY / L.w, IBZ / L.w};
     for (int d = 0; d < 3; ++d) grid *= (ncell + g[d] - 1) / g[d];
     hipLaunchKernelGGL((k_interp_cells<R, IZ>), dim3((unsigned)grid), dim3(IT), 0, s, L, boxes, (const XT<IZ>*)xp,
-                       vp, rhoc, off, phi, a_mid, dt, out);
+                       vp, rhoc, off, phi, kc, out);
     CUBE_HIP(hipGetLastError());
     return;
   }
-  launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
+  launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
 }
 
 }  // namespace
@@ -1278,10 +1278,10 @@ void pm1_deposit(int izipx, const PMLevel& L, const void* xp, const int32_t* rho
 
 template <typename R>
 void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp,
-               const int32_t* rhoc, const int64_t* off, const R* phi, float a_mid, float dt, PMOut* out,
+               const int32_t* rhoc, const int64_t* off, const R* phi, double kc, PMOut* out,
                hipStream_t s) {
-  if (izipx == 2) launch_interp<R, 2>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
-  else launch_interp<R, 1>(L, boxes, nbox, xp, vp, rhoc, off, phi, a_mid, dt, out, s);
+  if (izipx == 2) launch_interp<R, 2>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
+  else launch_interp<R, 1>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
 }
 
 template void pm_deposit<float>(int, const PMLevel&, const PMBox*, int, const void*, const int32_t*, const int64_t*,
@@ -1293,9 +1293,9 @@ template void pm1_deposit<float>(int, const PMLevel&, const void*, const int32_t
 template void pm1_deposit<double>(int, const PMLevel&, const void*, const int32_t*, const int64_t*, void*, double*,
                                   float, PMOut*, hipStream_t);
 template void pm_interp<float>(int, const PMLevel&, const PMBox*, int, const void*, float*, const int32_t*,
-                               const int64_t*, const float*, float, float, PMOut*, hipStream_t);
+                               const int64_t*, const float*, double, PMOut*, hipStream_t);
 template void pm_interp<double>(int, const PMLevel&, const PMBox*, int, const void*, float*, const int32_t*,
-                                const int64_t*, const double*, float, float, PMOut*, hipStream_t);
+                                const int64_t*, const double*, double, PMOut*, hipStream_t);
 
 void green_device(int n, float apm_soft, float apm_range, float gridsize, double factor, float* out, hipStream_t s) {
   int64_t tot = (int64_t)(n / 2 + 1) * n * n;

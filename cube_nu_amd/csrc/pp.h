@@ -17,7 +17,8 @@ struct PPLevel {
   float inv_app, inv_range;  // 1/app, 1/pp_range
   float two_inv_range;       // 2/pp_range
   float app2;                // app^2
-  float mass_scale, v_scale;
+  float mass_scale;
+  double v_scale;  // a_mid*dt/6/pi in FP64 (kick.f90:574 forms it in real(4))
   float inv_rcs;   // 1/rcs: fine -> coarse units (exact for a power-of-two rcs)
   float rwin;      // kSX * (r_pp in PP-cell units + margin): x window half-width in sub-bins
   float rcut2;     // (r_pp in PP-cell units + margin)^2: y/z column cut

@@ -365,13 +365,14 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
       }
     npg += (unsigned)nghost;
     npp += (unsigned)(npair - nghost);
-    // update v (kick.f90:576-582) in FP32 as the reference does
+    // update v (kick.f90:576-582): g = fl32(f mass_scale) as the reference
+    // forms it, times the FP64 coefficient, added in FP64
     const unsigned ip = __float_as_uint(me.w);
     float g[3];
     for (int d = 0; d < 3; ++d) g[d] = (float)f1[d] * L.mass_scale;
     f2 = fmaxf(f2, (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
     for (int d = 0; d < 3; ++d) {
-      float v = vp[3 * (int64_t)ip + d] + g[d] * L.v_scale;
+      const float v = (float)((double)vp[3 * (int64_t)ip + d] + (double)g[d] * L.v_scale);
       vp[3 * (int64_t)ip + d] = v;
       vm[d] = fmaxf(vm[d], fabsf(v));
     }

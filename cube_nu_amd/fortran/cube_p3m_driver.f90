@@ -96,7 +96,7 @@ program cube_p3m_driver
   open(newunit=u, file=trim(dir)//'/idx_b_r.bin', access='stream', form='unformatted', status='old'); read(u) idx_b_r
   close(u)
 
-  kin%a_mid = a_mid; kin%dt = dt; kin%mass_p_cdm = 1.0; kin%phase_mask = 0
+  kin%a_mid = real(a_mid, c_double); kin%dt = real(dt, c_double); kin%mass_p_cdm = 1.0; kin%phase_mask = 0
   call cube_check(cube_p3m_kick(ctx, xp, vp, rhoc, idx_b_r, np, kin, kout), ctx, 'kick')
   open(newunit=u, file=trim(dir)//'/vp_kick.bin', access='stream', form='unformatted', status='replace')
   write(u) vp; close(u)
@@ -107,7 +107,7 @@ program cube_p3m_driver
   close(u)
   call cube_check(cube_p3m_buffer_v(ctx, vp, pid, rhoc, np), ctx, 'buffer_v')
 
-  din%dt_old = dt_old; din%dt = dt_drift; din%vz_max = kout%vz_max
+  din%dt_old = real(dt_old, c_double); din%dt = real(dt_drift, c_double); din%vz_max = kout%vz_max
   call cube_check(cube_p3m_drift(ctx, xp, vp, pid, rhoc, np, idx_b_l, idx_b_r, sp(:, 1), sp(:, 2), sp(:, 3), &
                                  sp(:, 4), sp(:, 5), sp(:, 6), din, dout), ctx, 'drift')
   open(newunit=u, file=trim(dir)//'/xp_drift.bin', access='stream', form='unformatted', status='replace')

@@ -29,7 +29,8 @@ module cube_p3m_iface
   end type
 
   type, bind(C) :: cube_kick_in
-    real(c_float) :: a_mid, dt, mass_p_cdm
+    real(c_double) :: a_mid, dt        ! FP64 step coefficients (the module's real(4) values widen exactly)
+    real(c_float) :: mass_p_cdm
     integer(c_int32_t) :: phase_mask
   end type
 
@@ -42,7 +43,8 @@ module cube_p3m_iface
   end type
 
   type, bind(C) :: cube_drift_in
-    real(c_float) :: dt_old, dt, vz_max
+    real(c_double) :: dt_old, dt
+    real(c_float) :: vz_max
   end type
 
   type, bind(C) :: cube_drift_out

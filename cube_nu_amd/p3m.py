@@ -61,7 +61,7 @@ class Config(C.Structure):
 
 
 class KickIn(C.Structure):
-    _fields_ = [("a_mid", C.c_float), ("dt", C.c_float), ("mass_p_cdm", C.c_float), ("phase_mask", C.c_int32)]
+    _fields_ = [("a_mid", C.c_double), ("dt", C.c_double), ("mass_p_cdm", C.c_float), ("phase_mask", C.c_int32)]
 
 
 class KickOut(C.Structure):
@@ -80,7 +80,7 @@ class KickOut(C.Structure):
 
 
 class DriftIn(C.Structure):
-    _fields_ = [("dt_old", C.c_float), ("dt", C.c_float), ("vz_max", C.c_float)]
+    _fields_ = [("dt_old", C.c_double), ("dt", C.c_double), ("vz_max", C.c_float)]
 
 
 class DriftOut(C.Structure):

@@ -73,8 +73,8 @@ typedef struct cube_p3m_config {
 
 /* Kick inputs: the module scalars kick.f90 reads (variables.f90:35, sim). */
 typedef struct cube_kick_in {
-  float a_mid;
-  float dt;
+  double a_mid;        /* FP64: the kick coefficient a_mid*dt/6/pi is formed in FP64 */
+  double dt;
   float mass_p_cdm;
   int32_t phase_mask;  /* bit0 PM1, bit1 PM2, bit2 PM3, bit3 PP; 0 = all */
 } cube_kick_in;
@@ -90,8 +90,8 @@ typedef struct cube_kick_out {
 } cube_kick_out;
 
 typedef struct cube_drift_in {
-  float dt_old;        /* variables.f90:35 */
-  float dt;
+  double dt_old;       /* variables.f90:35 (real(4) there; FP64 here: dt_mid and  */
+  double dt;           /* the drift displacement dt_mid*v are formed in FP64)     */
   float vz_max;        /* sim%vz_max from the previous kick (kick.f90:315) */
 } cube_drift_in;
 

@@ -7,7 +7,11 @@
 //
 // Arithmetic follows the reference's FP32 rounding order (compiled with
 // -ffp-contract=off): decode and TSC positions in FP64, weights and
-// accumulations in FP32, kick factor as particle_mesh.f90:205, PP in FP32.
+// accumulations in FP32, PP in FP32.  One deliberate difference: the kick
+// and drift coefficients (a_mid*dt/6/pi, dt_mid = (dt_old+dt)/2 and the
+// displacement dt_mid*v) are formed in FP64, where the reference rounds them
+// to real(4) (particle_mesh.f90:205, kick.f90:574, 797, drift.f90:21, 57,
+// 117), because the implementation under test keeps them FP64.
 // FFTs are computed in FP64 (the reference's FFTW/MKL FP32 rounding is not
 // reproducible; see the DFT-definition tests).  Loop orders follow the
 // reference exactly, including the plane-colour passes that fix summation
@@ -44,7 +48,8 @@ struct OParams {
 };
 
 struct OKickIn {
-  float a_mid, dt, mass_p_cdm;
+  double a_mid, dt;  // FP64 step coefficients (real(4) in the reference, variables.f90:35)
+  float mass_p_cdm;
   int32_t phase_mask;
 };
 
@@ -57,7 +62,8 @@ struct OKickOut {
 };
 
 struct ODriftIn {
-  float dt_old, dt, vz_max;
+  double dt_old, dt;
+  float vz_max;
 };
 struct ODriftOut {
   int64_t nplocal;
@@ -151,11 +157,10 @@ const FDW FD;
 inline float g_grid() { return (float)((double)(1.0f / 6.0f) / PI); }
 
 // Kick update, particle_mesh.f90:205 / kick.f90:797:
-// vp = vp + dv*a_mid*dt/6/pi, with the /pi and the add in FP64.
-inline float kick_update(float vp, float dv, float a_mid, float dt) {
-  float t = ((dv * a_mid) * dt) / 6.0f;
-  return (float)((double)vp + (double)t / PI);
-}
+// vp = vp + dv*a_mid*dt/6/pi.  The coefficient a_mid*dt/6/pi is kept in FP64
+// (the reference forms dv*a_mid*dt in real(4)); product and add in FP64.
+inline double kick_coef(double a_mid, double dt) { return ((a_mid * dt) / 6.0) / PI; }
+inline float kick_update(float vp, float dv, double kc) { return (float)((double)vp + (double)dv * kc); }
 
 // ---------------------------------------------------------------------------
 // FFT (FP64 Stockham, any radix; used for the r2c/c2r 3-D transforms).
@@ -383,7 +388,7 @@ void fd_gradient(int64_t nforce, PHI phi, ForceField& F, float* f2max) {
 }
 
 // TSC gather + kick for one particle (particle_mesh.f90:178-208).
-inline void gather_kick(ForceField& F, double xpos[3], float* v, float a_mid, float dt,
+inline void gather_kick(ForceField& F, double xpos[3], float* v, double kc,
                         float vmax[3]) {
   float w[3][3];
   int64_t i0[3];
@@ -395,7 +400,7 @@ inline void gather_kick(ForceField& F, double xpos[3], float* v, float a_mid, fl
         for (int d = 0; d < 3; ++d)
           dv[d] = dv[d] + ((F.at(d, i0[0] + i1, i0[1] + i2, i0[2] + i3) * w[0][i1]) * w[1][i2]) * w[2][i3];
   for (int d = 0; d < 3; ++d) {
-    v[d] = kick_update(v[d], dv[d], a_mid, dt);
+    v[d] = kick_update(v[d], dv[d], kc);
     vmax[d] = std::max(vmax[d], std::fabs(v[d]));
   }
 }
@@ -489,13 +494,13 @@ void pm1(const Geo& g, const void* xp, float* vp, const int32_t* rhoc, const std
               xpos[d] = (double)((it[d] - 1) * nt + (ijk[d] - 1)) + decode(xp, ip, d, g);
               xpos[d] = xpos[d] * (double)g.rcs / gridsize;
             }
-            gather_kick(F, xpos, vp + 3 * ip, kin.a_mid, kin.dt, vm);
+            gather_kick(F, xpos, vp + 3 * ip, kick_coef(kin.a_mid, kin.dt), vm);
           }
         }
     for (int d = 0; d < 3; ++d) atomic_max_f(&vmax[d], vm[d]);
   }
   for (int d = 0; d < 3; ++d) out.vmax[d] = vmax[d];
-  out.dt_pm1 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2max) * kin.a_mid) * g_grid()));
+  out.dt_pm1 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2max) * (float)kin.a_mid) * g_grid()));
 }
 
 // One tile or sub-tile of PM2/PM3 (pm_thread, kick.f90:593-837).
@@ -578,7 +583,7 @@ void pm_box(const Geo& g, const void* xp, float* vp, const int32_t* rhoc, const 
             xpos[d] = (double)(ijk[d] - 1) + decode(xp, ip, d, g);
             xpos[d] = xpos[d] * (double)g.rcs / b.gridsize - (double)b.ushift[d];
           }
-          gather_kick(F, xpos, vp + 3 * ip, kin.a_mid, kin.dt, vmax_t);
+          gather_kick(F, xpos, vp + 3 * ip, kick_coef(kin.a_mid, kin.dt), vmax_t);
         }
       }
 }
@@ -764,14 +769,14 @@ void pp_box(const Geo& g, const OParams& P, const void* xp, float* vp, const int
     mass_scale = (float)((double)kin.mass_p_cdm * ((1.0 - P.f_nu) * (1.0 - P.f_nu)));
   else
     mass_scale = kin.mass_p_cdm;
-  float v_scale = (float)((double)((kin.a_mid * kin.dt) / 6.0f) / PI);
+  const double v_scale = kick_coef(kin.a_mid, kin.dt);  // FP64 (kick.f90:574 forms it in real(4))
   for (int64_t p = 0; p < nphy; ++p) {
     float f1[3];
     for (int d = 0; d < 3; ++d) f1[d] = af[3 * p + d] * mass_scale;
     *f2max_t = std::max(*f2max_t, (f1[0] * f1[0] + f1[1] * f1[1]) + f1[2] * f1[2]);
     int64_t ip = iploc[p];
     for (int d = 0; d < 3; ++d) {
-      vp[3 * ip + d] = vp[3 * ip + d] + f1[d] * v_scale;
+      vp[3 * ip + d] = (float)((double)vp[3 * ip + d] + (double)f1[d] * v_scale);
       vmax_t[d] = std::max(vmax_t[d], std::fabs(vp[3 * ip + d]));
     }
   }
@@ -894,7 +899,7 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
       for (int d = 0; d < 3; ++d) atomic_max_f(&vm[d], vt[d]);
     }
     out->f2max_pm2 = f2;
-    out->dt_pm2 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2) * kin->a_mid) * gg));
+    out->dt_pm2 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2) * (float)kin->a_mid) * gg));
     for (int d = 0; d < 3; ++d) vmax[d] = vm[d];
   }
   int64_t ns3 = g.ntiles() * g.nns * g.nns * g.nns;
@@ -913,7 +918,7 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
       for (int d = 0; d < 3; ++d) atomic_max_f(&vm[d], vt[d]);
     }
     out->f2max_pm3 = f2;
-    out->dt_pm3 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2) * kin->a_mid) * gg));
+    out->dt_pm3 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2) * (float)kin->a_mid) * gg));
     for (int d = 0; d < 3; ++d) vmax[d] = vm[d];
   }
   int64_t totpairs = 0;
@@ -931,7 +936,7 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
       for (int d = 0; d < 3; ++d) atomic_max_f(&vm[d], vt[d]);
     }
     out->f2max_pp = f2;
-    out->dt_pp = 0.05f * std::sqrt(1.0f / ((std::sqrt(f2) * kin->a_mid) * gg));
+    out->dt_pp = 0.05f * std::sqrt(1.0f / ((std::sqrt(f2) * (float)kin->a_mid) * gg));
     for (int d = 0; d < 3; ++d) vmax[d] = vm[d];
   }
   if (npairs_out) *npairs_out = totpairs;
@@ -1025,8 +1030,9 @@ int orc_drift(const OParams* P, const void* xp, const float* vp, const int64_t* 
   Geo g(*P);
   const int64_t nt = g.nt, ncb = g.ncb, nnt = g.nnt;
   std::vector<int64_t> off = cell_offsets(g, rhoc);
-  float dt_mid = (in->dt_old + in->dt) / 2.0f;  // drift.f90:21
-  int nlayer = 2 * (int)std::ceil((dt_mid * in->vz_max) / (float)g.rcs) + 2;  // drift.f90:26
+  // drift.f90:21 and :26, with the coefficient dt_mid in FP64
+  const double dt_mid = (in->dt_old + in->dt) / 2.0;
+  const int nlayer = 2 * (int)std::ceil((dt_mid * (double)in->vz_max) / (double)g.rcs) + 2;
   out->nlayer = nlayer;
   const int64_t ne = nt + 4 * ncb;  // rhoce(1-2ncb:nt+2ncb)
   auto RE = [&](int64_t i, int64_t j, int64_t k) { return ((k - 1 + 2 * ncb) * ne + (j - 1 + 2 * ncb)) * ne + (i - 1 + 2 * ncb); };
@@ -1046,7 +1052,7 @@ int orc_drift(const OParams* P, const void* xp, const float* vp, const int64_t* 
     auto dest = [&](int64_t ip, const int64_t ijk[3], int64_t gi[3]) -> bool {
       bool ok = true;
       for (int d = 0; d < 3; ++d) {
-        double dx = (double)((dt_mid * vp[3 * ip + d]) / (float)g.rcs);
+        double dx = (dt_mid * (double)vp[3 * ip + d]) / (double)g.rcs;
         double xq = ((double)ijk[d] - 1.0) + decode(xp, ip, d, g);
         gi[d] = (int64_t)std::ceil(xq + dx);
         if (gi[d] < 1 - 2 * ncb || gi[d] > nt + 2 * ncb) ok = false;
@@ -1093,8 +1099,8 @@ int orc_drift(const OParams* P, const void* xp, const float* vp, const int64_t* 
               int64_t idx = cst[e] + rholocal[e];
               rholocal[e]++;
               for (int d = 0; d < 3; ++d) {
-                float wv = dt_mid * vp[3 * ip + d];
-                double sh = (double)wv / (g.xres * (double)g.rcs);
+                const double wv = dt_mid * (double)vp[3 * ip + d];
+                double sh = wv / (g.xres * (double)g.rcs);
                 int64_t dn = (int64_t)std::round(sh);  // nint: half away from zero
                 int64_t xv = (g.izipx == 2) ? (int64_t)((const int16_t*)xp)[3 * ip + d] : (int64_t)((const int8_t*)xp)[3 * ip + d];
                 xnew[3 * idx + d] = xv + dn;

@@ -26,7 +26,7 @@ class OParams(C.Structure):
 
 
 class OKickIn(C.Structure):
-    _fields_ = [("a_mid", C.c_float), ("dt", C.c_float), ("mass_p_cdm", C.c_float), ("phase_mask", C.c_int32)]
+    _fields_ = [("a_mid", C.c_double), ("dt", C.c_double), ("mass_p_cdm", C.c_float), ("phase_mask", C.c_int32)]
 
 
 class OKickOut(C.Structure):
@@ -45,7 +45,7 @@ class OKickOut(C.Structure):
 
 
 class ODriftIn(C.Structure):
-    _fields_ = [("dt_old", C.c_float), ("dt", C.c_float), ("vz_max", C.c_float)]
+    _fields_ = [("dt_old", C.c_double), ("dt", C.c_double), ("vz_max", C.c_float)]
 
 
 class ODriftOut(C.Structure):

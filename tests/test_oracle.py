@@ -264,10 +264,12 @@ def test_buffer_v_restores_the_periodic_copies():
 # ---------------------------------------------------------------------- drift
 def _drift_spec(geo, xp, vp, pid, rhoc, dt_old, dt, vz_max):
     """Drift from the written spec (drift.f90:21-136): destination cell,
-    re-encode and the within-cell order key, computed with numpy sorts."""
+    re-encode and the within-cell order key, computed with numpy sorts.  The
+    coefficient dt_mid and the displacement dt_mid*v are FP64 (the reference
+    rounds both to real(4))."""
     B = 8 * geo.izipx
-    dt_mid = np.float32((np.float32(dt_old) + np.float32(dt)) / np.float32(2))
-    nlayer = 2 * int(np.ceil(np.float32(np.float32(dt_mid * np.float32(vz_max)) / np.float32(4)))) + 2
+    dt_mid = (float(dt_old) + float(dt)) / 2.0
+    nlayer = 2 * int(np.ceil(dt_mid * float(np.float32(vz_max)) / 4.0)) + 2
     nte, b, nt = geo.nte, geo.ncb, geo.nt
     r = rhoc.reshape(-1)
     n = int(r.sum())
@@ -277,11 +279,11 @@ def _drift_spec(geo, xp, vp, pid, rhoc, dt_old, dt, vz_max):
     c = cell % nte ** 3
     ijk = np.stack([c % nte, (c // nte) % nte, c // nte ** 2], 1) - b + 1  # Fortran tile-local
     frac = orc.decode(xp, geo.izipx)
-    v32 = (dt_mid * vp[:n]).astype(np.float32)
-    delta = (v32 / np.float32(4)).astype(np.float64)
+    v64 = dt_mid * vp[:n].astype(np.float64)
+    delta = v64 / 4.0
     g = np.ceil((ijk - 1) + frac[:n] + delta).astype(np.int64)
     assert g.min() >= 1 - 2 * b and g.max() <= nt + 2 * b
-    shift = v32.astype(np.float64) * 2.0 ** B / 4
+    shift = v64 * 2.0 ** B / 4
     nint = np.sign(shift) * np.floor(np.abs(shift) + 0.5)
     xnew = ((xp[:n].astype(np.int64) + nint.astype(np.int64) + 2 ** (B - 1)) % 2 ** B - 2 ** (B - 1))
     keep = ((g >= 1) & (g <= nt)).all(1)
@@ -313,8 +315,8 @@ def test_drift_matches_spec_and_invariants(izipx, dt):
 
 
 def test_drift_moves_particles_by_dt_v():
-    """Re-decoded global positions equal old position + fl32(dt_mid v)/4 to
-    within one coordinate bin (periodic)."""
+    """Re-decoded global positions equal old position + dt_mid v/4 to within
+    one coordinate bin (periodic)."""
     geo = Geometry(64)
     rng = np.random.default_rng(5)
     n = 3000
@@ -336,7 +338,7 @@ def test_drift_moves_particles_by_dt_v():
     gc = np.concatenate(cells)
     newpos = gc + orc.decode(xo, geo.izipx)
     old = pos[po - 1]
-    exp = old + (np.float32(0.5) * vel[po - 1]).astype(np.float32).astype(np.float64) / 4
+    exp = old + 0.5 * vel[po - 1].astype(np.float64) / 4
     d = (newpos - exp + geo.nc / 2) % geo.nc - geo.nc / 2
     assert np.abs(d).max() <= 1.5 / 2 ** 16
 

@@ -15,6 +15,7 @@
 // associative, so every output is independent of thread order.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/cube_p3m.h"
@@ -22,6 +23,10 @@
 #include "common.h"
 #include "fft.h"
 #include "pm.h"
+
+// The allocation-free CIC entry point's device scratch (sized and explained
+// with kScratchCells below).
+static __device__ __align__(256) unsigned long long scratch[(int64_t)1 << 23];
 
 namespace cube {
 namespace {
@@ -51,6 +56,34 @@ struct PMBuf {
 };
 
 constexpr double kFix = 4294967296.0;  // 2^32
+
+// Fixed device scratch of cube_pm_cic_noalloc: the fixed-point CIC
+// accumulator (batch * nw^3 64-bit cells, at most kScratchCells = 64 MiB).
+// Each GPU the library runs on holds its own copy of the array, at its own
+// address, until the process ends.  In host code `scratch` names a host-side
+// stand-in, never the device memory: host code uses only the per-device
+// address from hipGetSymbolAddress (scratch_address below).
+constexpr int64_t kScratchCells = (int64_t)1 << 23;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+unsigned long long* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(CUBE_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    static_assert(sizeof(::scratch) == sizeof(unsigned long long) * kScratchCells, "scratch size");
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess || !ptr)
+      throw Error(CUBE_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (unsigned long long*)ptr;
+  }
+  return g_scratch_ptr[device];
+}
 
 __device__ inline int wrapi(int i, int n) {
   i %= n;
@@ -393,6 +426,7 @@ int cube_pm_create(const cube_pm_config* cfg, cube_pm_ctx** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(CUBE_ERR_NO_DEVICE, "no HIP device");
     CUBE_HIP(hipGetDevice(&c->device));
+    (void)scratch_address(c->device);  // the allocation-free entry point's scratch, once per device
     const int n = cfg->n, B = cfg->batch;
     c->n3 = (int64_t)n * n * n;
     c->fc.init(n);
@@ -488,6 +522,28 @@ int cube_pm_cic(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float 
     hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->accw.p);
     hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3, 1.0 / 2147483648.0,
                        rho);
+    CUBE_HIP(hipGetLastError());
+  });
+}
+
+int cube_pm_cic_noalloc(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float scale, float* rho,
+                        void* stream) {
+  return pm_guarded(ctx, [&] {
+    ctx->check_np(np);
+    if (!x || !rho || nw < 2) throw Error(CUBE_ERR_ARG, "bad argument");
+    const int B = ctx->cfg.batch;
+    const int64_t nw3 = (int64_t)nw * nw * nw;
+    if (nw > 4096 || B * nw3 > kScratchCells)
+      throw Error(CUBE_ERR_ARG, "cube_pm_cic_noalloc: batch*nw^3 = " + std::to_string(B * nw3) +
+                                    " cells exceeds the fixed scratch of " + std::to_string(kScratchCells) +
+                                    " cells (64 MiB); use cube_pm_cic");
+    unsigned long long* acc = scratch_address(ctx->device);
+    const hipStream_t s = (hipStream_t)stream;
+    const int64_t ntot = np * B;
+    // the array keeps whatever the last call on this GPU left: clear it here
+    hipLaunchKernelGGL(k_zero_u64, dim3(nblk(B * nw3)), dim3(256), 0, s, acc, B * nw3);
+    hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, acc);
+    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, acc, B * nw3, 1.0 / 2147483648.0, rho);
     CUBE_HIP(hipGetLastError());
   });
 }

@@ -32,7 +32,7 @@ from .p3m import CubeError, load_library
 
 PM_ABI_FUNCTIONS = (
     "cube_pm_create", "cube_pm_destroy", "cube_pm_last_error", "cube_pm_green", "cube_pm_force",
-    "cube_pm_force_vjp", "cube_pm_cic", "cube_pm_cic_vjp",
+    "cube_pm_force_vjp", "cube_pm_cic", "cube_pm_cic_vjp", "cube_pm_cic_noalloc",
 )
 
 
@@ -54,6 +54,7 @@ def _lib():
     lib.cube_pm_force_vjp.argtypes = [P, vp, i64, vp, vp, vp, vp]
     lib.cube_pm_cic.argtypes = [P, vp, i64, C.c_int32, C.c_float, vp, vp]
     lib.cube_pm_cic_vjp.argtypes = [P, vp, i64, C.c_int32, C.c_float, vp, vp, vp]
+    lib.cube_pm_cic_noalloc.argtypes = [P, vp, i64, C.c_int32, C.c_float, vp, vp]
     lib._pm_bound = True
     return lib
 

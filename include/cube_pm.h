@@ -69,6 +69,18 @@ int cube_pm_force_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, const float*
  * (cell i covers [i, i+1), weights 1-d, d): rho (B*nw^3). */
 int cube_pm_cic(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float scale, float* rho, void* stream);
 
+/* cube_pm_cic without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * fixed-point accumulator lives in a fixed device array of the library,
+ * 2^23 64-bit cells (64 MiB) per GPU, held until the process ends; calls with
+ * batch*nw^3 above 2^23 return CUBE_ERR_ARG.  Every call on one GPU shares
+ * that array (each call clears it first), so calls that overlap on one GPU,
+ * from any context or stream, race: order them.  Same result as cube_pm_cic,
+ * bit for bit. */
+int cube_pm_cic_noalloc(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float scale, float* rho,
+                        void* stream);
+
 /* g_x = (d rho / d x)^T g_rho for cube_pm_cic.  g_x is overwritten. */
 int cube_pm_cic_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float scale, const float* g_rho,
                     float* g_x, void* stream);

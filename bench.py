@@ -15,6 +15,7 @@ the dominant kernel (HIP-event durations on the library stream) and a
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -56,8 +57,12 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
         p.timing_reset()
     t0 = time.perf_counter()
     nloc = 0
+    per_step, pairs = [], 0
     for s in range(steps):
+        ts = time.perf_counter()
         d, k = p.step_resident(dt_old=dt, dt=dt, vz_max=vz, a_mid=a)
+        per_step.append(time.perf_counter() - ts)  # the step returns after its closing read-back
+        pairs += p.timing_report()["pp_pairs_last"]["calls"]
         vz = k["vz_max"]
         nloc = d["nplocal"]
     p.synchronize()
@@ -66,7 +71,8 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
     tuned = {k: v["calls"] for k, v in p.timing_report().items() if k.startswith("pp_unroll_level")}
     p.close()
     log(f"[bench] fp{precision}: {steps} steps in {t:.3f}s ({1e3 * t / steps:.1f} ms/step)")
-    return {"seconds": t, "n": nloc, "report": rep, "dt": dt, "autotune": tuned}
+    return {"seconds": t, "n": nloc, "report": rep, "dt": dt, "autotune": tuned, "per_step": per_step,
+            "pairs": pairs}
 
 
 def measured_traffic(kernel, ng):
@@ -85,7 +91,7 @@ def measured_traffic(kernel, ng):
     return None
 
 
-def roofline(rep, steps, ng):
+def roofline(rep, steps, ng, pairs_total=None):
     kern = {k: v for k, v in rep.items() if v["ms"] > 0 and k != "green"}
     if not kern:
         return None, {}
@@ -95,7 +101,9 @@ def roofline(rep, steps, ng):
     phases = {k: round(vv["ms"] / steps, 3) for k, vv in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
     if name == "pp_force":
         # 40 FP32 flops per interacting pair, each unordered pair once
-        pairs = rep.get("pp_pairs_last", {}).get("calls", 0)
+        # pairs of the same timed steps as the kernel time
+        pairs = (pairs_total if pairs_total is not None else rep.get("pp_pairs_last", {}).get("calls", 0) * steps)
+        pairs = pairs / steps
         ach = 40.0 * pairs / (v["ms"] / steps * 1e-3) / 1e12
         r = {"kernel": name, "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
              "frac": ach / FP32_PEAK_TFLOPS, "traffic": measured_traffic(name, ng), "avg_launch_ms": avg_ms,
@@ -108,32 +116,67 @@ def roofline(rep, steps, ng):
     return r, phases
 
 
-def cpu_baseline(ng_sample, z, box):
-    """Oracle (C++/OpenMP restatement) timed on the host: one full step
-    (kick + drift) at ng_sample^3 with the same IC recipe."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ng, z, box, stride, steps=3):
+    """Oracle (C++/OpenMP restatement of the reference path, FP32-faithful)
+    timed on the host on a bounded sample of the same workload: the same
+    ng^3 Zel'dovich state, the kick's PM2, PM3 and PP loops and the drift run
+    on the tiles T with T % stride == 0 (PM1, a global mesh, runs whole);
+    one warm-up step, then the median of `steps` steps.  The step time is
+    scaled to the whole workload as t_PM1 + stride (t_PM2 + t_PM3 + t_PP +
+    t_drift); per-phase times are reported in the reference's tcat slots
+    (kick.f90:33-298, drift.f90:16)."""
+    import statistics
+
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from cube_nu_amd import P3M
     from inputs import oparams
     from oracle import pyoracle as orc
 
-    p = P3M(ng_sample, izipx=2, box=box * ng_sample / 512.0)
+    p = P3M(ng, izipx=2, box=box)
     p.make_ic(seed=20260101, z=z, nic=2)
-    xp, vp, pid, rhoc = p.download()
+    geo = p.geo
+    xp, vp, pid, rhoc = p.download(cap=int(ng ** 3 * (geo.nte / geo.nt) ** 3 * 1.15) + 4096)
     p.close()
-    P = oparams(p.geo)
+    P = oparams(geo)
     tfs = orc.tf_default(P)
     a = 1.0 / (1.0 + z)
-    v = vp.copy()
-    out, _, _ = orc.kick(P, xp, v, rhoc, tfs, a, 0.0)
+    out, _, _ = orc.kick(P, xp, vp.copy(), rhoc, tfs, a, 0.0)
     dt = min(out["dt_pm1"], out["dt_pm2"], out["dt_pm3"], out["dt_pp"], 1.0)
-    t0 = time.perf_counter()
-    out, _, _ = orc.kick(P, xp, v, rhoc, tfs, a, dt)
-    st, *_ = orc.drift(P, xp, v, pid, rhoc, dt, dt, out["vz_max"])
-    t = time.perf_counter() - t0
-    n = int(rhoc[:, :, :, 4:-4, 4:-4, 4:-4].sum())
-    return {"value": n / t, "unit": "particle-updates/s", "cores": orc.num_threads(), "kind": "port",
-            "sample": f"one full oracle step (kick PM1+PM2+PM3+PP + drift) at {ng_sample}^3 = {n} particles, "
-                      f"same Zel'dovich recipe, {t:.2f} s"}
+    orc.set_sample_stride(stride)
+    per = []
+    try:
+        for s in range(steps + 1):
+            v = vp.copy()
+            out, _, _ = orc.kick(P, xp, v, rhoc, tfs, a, dt)
+            orc.drift(P, xp, v, pid, rhoc, dt, dt, out["vz_max"])
+            t = orc.phase_seconds()
+            scaled = [t[0]] + [stride * x for x in t[1:]]
+            log(f"[bench] cpu baseline step {s}: phases {[round(x, 2) for x in t]} s, scaled {sum(scaled):.1f} s")
+            if s > 0:
+                per.append(scaled)
+    finally:
+        orc.set_sample_stride(1)
+    totals = [sum(x) for x in per]
+    med = statistics.median(totals)
+    k = totals.index(med) if med in totals else 0
+    n = ng ** 3
+    names = ("11_pm1", "12_pm2", "13_pm3", "14_pp", "2_drift")
+    return {"value": n / med, "unit": "particle-updates/s", "cores": orc.num_threads(), "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"the {ng}^3 z={z:g} state; kick PM2/PM3/PP and drift on the tiles T % {stride} == 0 "
+                      f"({64 // stride} of 64), PM1 whole; 1 warm-up + median of {steps} steps, scaled to all "
+                      f"tiles: {med:.1f} s/step",
+            "tcat_ms_per_step": {nm: round(1e3 * x, 1) for nm, x in zip(names, per[k])}}
 
 
 def copy_calibration(mib):
@@ -156,9 +199,15 @@ def pm_forward_adjoint(reps=2):
     a batch of 4 seeds of 128^3 particles on 256^3 periodic meshes, box
     256 Mpc/h, Zel'dovich from white noise, 10 KDK steps from a = 0.1 to 1,
     loss = MSE between the final CIC densities on 128^3 and the CIC density
-    of a fifth seed; gradient with respect to the 4 x 128^3 noise modes."""
+    of a fifth seed; gradient with respect to the 4 x 128^3 noise modes.
+    Particle positions are FP32 displacements from the particle lattice; the
+    backward recomputes each step's force mesh from checkpointed positions.
+    The gradient of seed 0 is compared with the FP64 plain-PyTorch model's
+    autograd gradient (torch's float64 GPU kernels), whose own directional
+    derivative is compared with centred finite differences of that model."""
     import torch
     from cube_nu_amd import pmdiff
+    from oracle import pm_torch as ref
 
     B, ng, n, nw, nsteps, box = 4, 128, 256, 128, 10, 256.0
     k1, d, k2 = pmdiff.kdk_coefficients(0.1, 1.0, nsteps)
@@ -166,18 +215,18 @@ def pm_forward_adjoint(reps=2):
     k1 = [c / (4 * 3.141592653589793 * rho_mean) for c in k1]
     k2 = [c / (4 * 3.141592653589793 * rho_mean) for c in k2]
     amp = 0.5
-    pm = pmdiff.PMForce(n, B, 1.0)
-    pm1 = pmdiff.PMForce(n, 1, 1.0)
+    pm = pmdiff.PMForce(n, B, 1.0, lattice=ng)
+    pm1 = pmdiff.PMForce(n, 1, 1.0, lattice=ng)
     gen = torch.Generator(device="cuda").manual_seed(20)
     noise = torch.randn(B, ng, ng, ng, device="cuda", generator=gen)
     z5 = torch.randn(1, ng, ng, ng, device="cuda", generator=gen)
     with torch.no_grad():
-        x, v = pmdiff.zeldovich(z5, box, n, amp)
+        x, v = pmdiff.zeldovich(z5, box, n, amp, displacement=True)
         x, v = pmdiff.kdk(x, v, pm1, k1, d, k2)
         target = pmdiff.cic_density(x, pm1, nw, nw / n)
 
     def loss_of(z):
-        x, v = pmdiff.zeldovich(z, box, n, amp)
+        x, v = pmdiff.zeldovich(z, box, n, amp, displacement=True)
         x, v = pmdiff.kdk(x, v, pm, k1, d, k2)
         rho = pmdiff.cic_density(x, pm, nw, nw / n)
         return ((rho.double() - target.double()) ** 2).mean()
@@ -200,27 +249,44 @@ def pm_forward_adjoint(reps=2):
         loss_of(z).backward()
         return z.grad
 
+    torch.cuda.reset_peak_memory_stats()
     t_f = timed(fwd)
+    mem_f = torch.cuda.max_memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
     t_fa = timed(fwd_adj)
-    # directional derivative against centred finite differences along a
-    # direction confined to a 16^3 block of one seed's noise (FP32 forward:
-    # the error has a truncation part ~eps^2 and a rounding part ~1/eps, so
-    # several step sizes are reported)
+    mem_fa = torch.cuda.max_memory_allocated()
     g = fwd_adj()
-    h = torch.zeros_like(noise)
-    h[0, :16, :16, :16] = torch.randn((16, 16, 16), device="cuda", generator=gen)
-    gd = float((g.double() * h.double()).sum())
-    fd_err = {}
-    with torch.no_grad():
-        for eps in (0.3, 0.1, 0.03, 0.01):
-            fd = float((loss_of(noise + eps * h) - loss_of(noise - eps * h)) / (2 * eps))
-            fd_err[f"{eps:g}"] = abs(gd - fd) / abs(fd)
     pm.close()
     pm1.close()
+    # seed 0 against the FP64 model (the seeds' meshes are independent, so
+    # seed 0's gradient of the batch loss is 1/B of its own loss's gradient)
+    G = torch.from_numpy(pmdiff.PMForce(n, 1, 1.0).green().astype("float64")).cuda()
+
+    def loss64(z):
+        x, v = pmdiff.zeldovich(z, box, n, amp)
+        x, v = ref.kdk(x, v, G, 1, n, k1, d, k2)
+        return ((ref.cic(x, 1, nw, nw / n) - target.double()) ** 2).mean()
+
+    z0 = noise[:1].double().clone().requires_grad_(True)
+    (g64,) = torch.autograd.grad(loss64(z0), z0)
+    g0 = g[:1].double() * B
+    grad_err = float(torch.linalg.vector_norm(g0 - g64) / torch.linalg.vector_norm(g64))
+    h = torch.zeros_like(z0)
+    h[0, :16, :16, :16] = torch.randn((16, 16, 16), device="cuda", generator=gen).double()
+    with torch.no_grad():
+        eps = 1e-4
+        fd = float((loss64(z0.detach() + eps * h) - loss64(z0.detach() - eps * h)) / (2 * eps))
+    gd64 = float((g64 * h).sum())
+    gd32 = float((g0 * h).sum())
     return {"config": "4 seeds x 128^3 particles, 256^3 mesh, box 256 Mpc/h, 10 KDK steps a=0.1->1, "
-                      "MSE of 128^3 CIC density vs a fifth seed, gradient w.r.t. the noise modes",
+                      "MSE of 128^3 CIC density vs a fifth seed, gradient w.r.t. the noise modes; positions as "
+                      "FP32 lattice displacements, per-step position checkpoints, force meshes recomputed "
+                      "in the backward",
             "forward_ms": t_f, "forward_adjoint_ms": t_fa, "adjoint_over_forward": t_fa / t_f,
-            "fd_directional_rel_err": fd_err}
+            "peak_bytes_forward": mem_f, "peak_bytes_forward_adjoint": mem_fa,
+            "grad_rel_err_vs_fp64_model": grad_err,
+            "fd_directional_rel_err": {"fp32_gradient_vs_fp64_fd": abs(gd32 - fd) / abs(fd),
+                                       "fp64_gradient_vs_fp64_fd": abs(gd64 - fd) / abs(fd), "eps": eps}}
 
 
 def _rel(a, b, chunk=1 << 24):
@@ -322,9 +388,10 @@ def main():
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--z", type=float, default=2.0)
     ap.add_argument("--box", type=float, default=600.0)
-    ap.add_argument("--fp64-steps", type=int, default=1,
+    ap.add_argument("--fp64-steps", type=int, default=3,
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
-    ap.add_argument("--cpu-sample-ng", type=int, default=384, help="0 = skip the CPU baseline")
+    ap.add_argument("--cpu-stride", type=int, default=8,
+                    help="CPU baseline sample: tiles T %% stride == 0 of the same state (0 = skip the CPU baseline)")
     ap.add_argument("--pm-adjoint", type=int, default=1, help="0 = skip the PM forward+adjoint timing")
     ap.add_argument("--precision-study", type=int, default=1,
                     help="0 = skip the per-field FP32-vs-FP64 comparison")
@@ -354,14 +421,20 @@ def main():
         sys.exit(2)
     r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box)
     value = r32["n"] * args.steps / r32["seconds"]
-    roof, phases = roofline(r32["report"], args.steps, args.ng)
+    roof, phases = roofline(r32["report"], args.steps, args.ng, r32["pairs"])
     fp64 = fp64_plain = None
+    med32 = statistics.median(r32["per_step"])
     if args.fp64_steps > 0:
         def form(per_tile):
             try:
-                r = run_form(args.ng, 64, args.fp64_steps, 0, args.z, args.box, timing=False, per_tile=per_tile)
+                r = run_form(args.ng, 64, args.fp64_steps, args.warmup if not per_tile else 1, args.z, args.box,
+                             timing=False, per_tile=per_tile)
                 v = r["n"] * args.fp64_steps / r["seconds"]
-                return {"value": v, "ms_per_step": 1e3 * r["seconds"] / args.fp64_steps, "fp32_speedup": value / v}
+                med = statistics.median(r["per_step"])
+                return {"value": v, "ms_per_step": 1e3 * r["seconds"] / args.fp64_steps,
+                        "median_ms_per_step": 1e3 * med, "steps": args.fp64_steps,
+                        "warmup": args.warmup if not per_tile else 1,
+                        "fp32_speedup": value / v, "fp32_speedup_median": med / med32}
             except Exception as e:  # reported, not hidden
                 return {"error": str(e)}
         fp64 = form(False)
@@ -379,14 +452,15 @@ def main():
         except Exception as e:  # reported, not hidden
             prec = {"error": str(e)}
     cpu = None
-    if args.cpu_sample_ng > 0:
+    if args.cpu_stride > 0:
         try:
-            cpu = cpu_baseline(args.cpu_sample_ng, args.z, args.box)
+            cpu = cpu_baseline(args.ng, args.z, args.box, args.cpu_stride)
         except Exception as e:
             cpu = {"error": str(e)}
     line = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / args.steps, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+        "median_ms_per_step": 1e3 * med32, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded Zel'dovich IC (BBKS P(k), sigma8=0.8) generated on the GPU",
         "config": {"workload": f"{args.ng}^3 CDM, box {args.box:g} Mpc/h, z={args.z:g}, full P3M step "

@@ -206,3 +206,39 @@ def choose_step(hist: ExpansionHistory, dt_pm1, dt_pm2, dt_pm3, dt_pp, dt_vmax, 
     a_mid = f(hist.a + f(da / 2))
     dtau = hist.dtau_a(da)
     return dt, da, dtau, a_mid, landed
+
+
+def fixed_step_schedule(hist: ExpansionHistory, z_end, nsteps):
+    """A fixed code-time step dt that takes `nsteps` steps from hist.a to
+    a_end = 1/(1+z_end), and the per-step (dt, a, da, a_mid, dtau) of the
+    main loop with that dt (timestep.f90:25-89 with dt fixed instead of the
+    minimum of the kick's criteria).  The table's a(t) is piecewise linear, so
+    dt = (t(a_end) - t(a)) / nsteps; the last step lands on a_end by the
+    reference's rescaling rule (timestep.f90:57-61).  hist is advanced to
+    a_end."""
+    f = np.float32
+    a_end = f(1.0 / (1.0 + z_end))
+
+    def code_time(a8):
+        stime, s2a = hist.stime, hist.s2a
+        i1 = 1
+        while s2a[i1 + 1] >= a8 and i1 < hist.istep_max - 1:
+            i1 += 1
+        tdoa = (float(stime[i1 + 1]) - float(stime[i1])) / (float(s2a[i1 + 1]) - float(s2a[i1]))
+        return float(stime[i1]) + tdoa * (float(a8) - float(s2a[i1]))
+
+    dt = f((code_time(float(a_end)) - code_time(float(hist.a))) / nsteps)
+    steps = []
+    for n in range(nsteps):
+        d = f(dt)
+        da = hist.expansion(d)
+        if n == nsteps - 1 or da >= f(a_end - hist.a):
+            while abs(f(hist.a + da) / a_end - 1) >= 1e-6:
+                d = f(d * f(f(a_end - hist.a) / da))
+                da = hist.expansion(d)
+        a_mid = f(hist.a + f(da / 2))
+        dtau = hist.dtau_a(da)
+        steps.append({"dt": float(d), "a": float(hist.a), "da": float(da), "a_mid": float(a_mid),
+                      "dtau": float(dtau)})
+        hist.advance(da, dtau)
+    return steps

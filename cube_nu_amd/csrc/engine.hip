@@ -115,6 +115,7 @@ struct cube_p3m_ctx {
   DevBuf<int32_t> ppcount;
   DevBuf<int64_t> ppstart;
   DevBuf<float4> ppent;
+  DevBuf<unsigned> ppclaim;
   DevBuf<int32_t> dflags, drank, dtable, dnewcount;
   DevBuf<int64_t> dbase;
   DevBuf<int32_t> physcount;
@@ -196,6 +197,8 @@ void cube_p3m_ctx::init() {
   if (c.ng <= 0 || c.nnt <= 0 || c.nns <= 0 || c.ratio_cs <= 0 || c.ncb <= 0 || c.ngb < 0)
     throw Error(CUBE_ERR_ARG, "non-positive size in configuration");
   if (c.izipx != 1 && c.izipx != 2) throw Error(CUBE_ERR_ARG, "izipx must be 1 or 2");
+  if (c.ires_fixed != 0 && (c.ires_fixed < 2 || c.ires_fixed > 5))
+    throw Error(CUBE_ERR_ARG, "ires_fixed must be 0 (adaptive) or 2..5");
   if (c.precision != 32 && c.precision != 64) throw Error(CUBE_ERR_ARG, "precision must be 32 or 64");
   if (c.ng % (c.ratio_cs * c.nnt * c.nns) != 0) throw Error(CUBE_ERR_ARG, "ng must be a multiple of ratio_cs*nnt*nns");
   prec = c.precision;
@@ -367,6 +370,7 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
       S.start = ppstart.ensure((size_t)(nb * m3 + 1));
       S.scan_tmp = scan_scratch(nb * m3);
       S.scan_tmp_bytes = scan_tmp.n;
+      S.claim = ppclaim.ensure(8);
       int64_t total;
       {
         KScope k(&tm, "pp_bin_count", 0);
@@ -484,7 +488,7 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
   // classification (kick.f90:108-138)
   std::vector<int32_t> ires_h(g.ns3, 2), cmax_h(g.ns3, 0);
   if (mask & 12) {
-    classify_subtiles(g, rhocc(), ires.p, napm.p, cmax.p, s);
+    classify_subtiles(g, cfg.ires_fixed, rhocc(), ires.p, napm.p, cmax.p, s);
     CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipMemcpyAsync(cmax_h.data(), cmax.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipMemcpyAsync(out.napm, napm.p, sizeof(int32_t) * 7, hipMemcpyDeviceToHost, s));

@@ -66,9 +66,10 @@ void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, co
 void green_device(int n, float apm_soft, float apm_range, float gridsize, double factor, float* out, hipStream_t s);
 
 // Sub-tile classification (kick.f90:110-133): ires(4096) and napm counts.
-// ires and the largest physical cell count (cmax) of every sub-tile.
-void classify_subtiles(const Geo& g, const int32_t* rhoc, int32_t* ires, int32_t* napm, int32_t* cmax,
-                       hipStream_t s);
+// ires and the largest physical cell count (cmax) of every sub-tile;
+// ires_fixed in 2..5 puts every sub-tile at that level instead.
+void classify_subtiles(const Geo& g, int ires_fixed, const int32_t* rhoc, int32_t* ires, int32_t* napm,
+                       int32_t* cmax, hipStream_t s);
 
 // Binned deposit for sub-tile levels with w <= 8 mesh cells per coarse cell
 // and a power-of-two ratio_cs/gridsize: one pass lists, for every 16^3 mesh

@@ -1016,7 +1016,7 @@ __global__ void k_green(int n, float apm_soft, float apm_range, float gridsize, 
 }
 
 // kick.f90:110-133: npc_max = maxval(rhoc(sub-tile))/64.
-__global__ void k_classify(Geo g, const int32_t* __restrict__ rhoc, int32_t* __restrict__ ires,
+__global__ void k_classify(Geo g, int ires_fixed, const int32_t* __restrict__ rhoc, int32_t* __restrict__ ires,
                            int32_t* __restrict__ napm, int32_t* __restrict__ cmax) {
   __shared__ int32_t red[256];
   const int64_t it = blockIdx.x;
@@ -1038,6 +1038,7 @@ __global__ void k_classify(Geo g, const int32_t* __restrict__ rhoc, int32_t* __r
   if (threadIdx.x == 0) {
     float npc = (float)red[0] / 64.0f;
     int r = npc < 24.0f ? 2 : (npc < 114.0f ? 3 : (npc < 320.0f ? 4 : 5));
+    if (ires_fixed >= 2) r = ires_fixed;  // kick.f90:131 (commented there)
     ires[it] = r;
     cmax[it] = red[0];
     atomicAdd(&napm[r - 1], 1);
@@ -1325,10 +1326,10 @@ void zero_async(void* p, size_t bytes, hipStream_t s) {
   CUBE_HIP(hipGetLastError());
 }
 
-void classify_subtiles(const Geo& g, const int32_t* rhoc, int32_t* ires, int32_t* napm, int32_t* cmax,
-                       hipStream_t s) {
+void classify_subtiles(const Geo& g, int ires_fixed, const int32_t* rhoc, int32_t* ires, int32_t* napm,
+                       int32_t* cmax, hipStream_t s) {
   zero_async(napm, sizeof(int32_t) * 7, s);
-  hipLaunchKernelGGL(k_classify, dim3((unsigned)g.ns3), dim3(256), 0, s, g, rhoc, ires, napm, cmax);
+  hipLaunchKernelGGL(k_classify, dim3((unsigned)g.ns3), dim3(256), 0, s, g, ires_fixed, rhoc, ires, napm, cmax);
   CUBE_HIP(hipGetLastError());
 }
 

@@ -113,6 +113,57 @@ __device__ inline void cic(float y, int& i, float w[2]) {
   w[1] = d;
 }
 
+// Particle positions: absolute mesh coordinates (L = 0), or displacements
+// from a Lagrangian lattice of L^3 sites per mesh (L > 0): particle p of a
+// batch sits at site q = p mod L^3 (x fastest) whose position along each
+// axis is (i + 1/2) n / L, plus its displacement.  With displacements the
+// floor index and the fraction come from the integer and fractional parts of
+// the site position (exact in FP64) plus the FP32 displacement, so their
+// rounding is that of the displacement (|s| of a few cells), not of the
+// absolute coordinate (up to n cells).
+struct Lattice {
+  int L = 0;          // sites per dimension; 0 = absolute positions
+  double step = 0.0;  // n / L mesh cells between sites
+};
+
+// Site position along one axis times `scale`, split into integer and fraction.
+__device__ inline void site_parts(const Lattice& lt, int64_t p, int d, double scale, int& ip, float& fp) {
+  const int64_t q = p % ((int64_t)lt.L * lt.L * lt.L);
+  const int64_t c = d == 0 ? q % lt.L : (d == 1 ? (q / lt.L) % lt.L : q / ((int64_t)lt.L * lt.L));
+  const double v = ((double)c + 0.5) * lt.step * scale;
+  const double f = floor(v);
+  ip = (int)f;
+  fp = (float)(v - f);
+}
+
+// TSC floor index and weights of coordinate d of particle p (p counts over
+// the whole batch; the site is p mod L^3).
+__device__ inline void tsc_pos(const float* __restrict__ x, int64_t p, int d, const Lattice& lt, int& i, float w[3],
+                               float dw[3]) {
+  if (lt.L == 0) {
+    tsc(x[3 * p + d], i, w, dw);
+    return;
+  }
+  int ip;
+  float fp;
+  site_parts(lt, p, d, 1.0, ip, fp);
+  tsc(fp + x[3 * p + d], i, w, dw);  // the part below one cell, then the integer offset
+  i += ip;
+}
+
+__device__ inline void cic_pos(const float* __restrict__ x, int64_t p, int d, const Lattice& lt, float scale, int& i,
+                               float w[2]) {
+  if (lt.L == 0) {
+    cic(x[3 * p + d] * scale, i, w);
+    return;
+  }
+  int ip;
+  float fp;
+  site_parts(lt, p, d, (double)scale, ip, fp);
+  cic(fp + x[3 * p + d] * scale, i, w);
+  i += ip;
+}
+
 __global__ void k_zero_u32(unsigned int* a, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = 0;
@@ -124,14 +175,14 @@ __global__ void k_zero_u64(unsigned long long* a, int64_t n) {
 }
 
 // rho += W (fixed point 2^-32); one thread per particle.
-__global__ void k_tsc_deposit(const float* __restrict__ x, int64_t np, int64_t ntot, int n,
+__global__ void k_tsc_deposit(const float* __restrict__ x, int64_t np, int64_t ntot, int n, Lattice lt,
                               unsigned long long* __restrict__ acc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
   const int64_t b = p / np, n3 = (int64_t)n * n * n;
   int i[3];
   float w[3][3], dw[3][3];
-  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  for (int d = 0; d < 3; ++d) tsc_pos(x, p, d, lt, i[d], w[d], dw[d]);
   unsigned long long* a = acc + b * n3;
   for (int k3 = 0; k3 < 3; ++k3) {
     const int z = wrapi(i[2] - 1 + k3, n);
@@ -197,14 +248,14 @@ __global__ void k_fd_adjoint(const float* __restrict__ Fb, int B, int n, float* 
 }
 
 // acc_p = sum W F.
-__global__ void k_tsc_interp(const float* __restrict__ x, int64_t np, int64_t ntot, int n,
+__global__ void k_tsc_interp(const float* __restrict__ x, int64_t np, int64_t ntot, int n, Lattice lt,
                              const float* __restrict__ F, float* __restrict__ acc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
   const int64_t b = p / np, n3 = (int64_t)n * n * n;
   int i[3];
   float w[3][3], dw[3][3];
-  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  for (int d = 0; d < 3; ++d) tsc_pos(x, p, d, lt, i[d], w[d], dw[d]);
   const float* Fb = F + b * 3 * n3;
   float a[3] = {0, 0, 0};
   for (int k3 = 0; k3 < 3; ++k3) {
@@ -251,14 +302,15 @@ __device__ inline double fix_scale(const unsigned int* gmax) {
 
 // Fbar[b][d] += W g_d in fixed point of scale S.
 __global__ void k_tsc_deposit_vec(const float* __restrict__ x, const float* __restrict__ g, int64_t np, int64_t ntot,
-                                  int n, const unsigned int* __restrict__ gmax, unsigned long long* __restrict__ acc) {
+                                  int n, Lattice lt, const unsigned int* __restrict__ gmax,
+                                  unsigned long long* __restrict__ acc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
   const double S = fix_scale(gmax);
   const int64_t b = p / np, n3 = (int64_t)n * n * n;
   int i[3];
   float w[3][3], dw[3][3];
-  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  for (int d = 0; d < 3; ++d) tsc_pos(x, p, d, lt, i[d], w[d], dw[d]);
   const float gs[3] = {(float)(g[3 * p] * S), (float)(g[3 * p + 1] * S), (float)(g[3 * p + 2] * S)};
   unsigned long long* a = acc + b * 3 * n3;
   for (int k3 = 0; k3 < 3; ++k3) {
@@ -289,7 +341,7 @@ __global__ void k_sfix_to_float(unsigned long long* __restrict__ acc, int64_t n,
 }
 
 // g_x = sum grad W (F.g + rhobar).
-__global__ void k_tsc_vjp_gather(const float* __restrict__ x, int64_t np, int64_t ntot, int n,
+__global__ void k_tsc_vjp_gather(const float* __restrict__ x, int64_t np, int64_t ntot, int n, Lattice lt,
                                  const float* __restrict__ F, const float* __restrict__ g,
                                  const float* __restrict__ rhob, float* __restrict__ gx) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,7 +349,7 @@ __global__ void k_tsc_vjp_gather(const float* __restrict__ x, int64_t np, int64_
   const int64_t b = p / np, n3 = (int64_t)n * n * n;
   int i[3];
   float w[3][3], dw[3][3];
-  for (int d = 0; d < 3; ++d) tsc(x[3 * p + d], i[d], w[d], dw[d]);
+  for (int d = 0; d < 3; ++d) tsc_pos(x, p, d, lt, i[d], w[d], dw[d]);
   const float gp[3] = {g[3 * p], g[3 * p + 1], g[3 * p + 2]};
   const float* Fb = F + b * 3 * n3;
   const float* rb = rhob + b * n3;
@@ -319,14 +371,14 @@ __global__ void k_tsc_vjp_gather(const float* __restrict__ x, int64_t np, int64_
 }
 
 // CIC density of x*scale on nw^3 (fixed point 2^-32).
-__global__ void k_cic_deposit(const float* __restrict__ x, int64_t np, int64_t ntot, int nw, float scale,
+__global__ void k_cic_deposit(const float* __restrict__ x, int64_t np, int64_t ntot, int nw, float scale, Lattice lt,
                               unsigned long long* __restrict__ acc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
   const int64_t b = p / np, n3 = (int64_t)nw * nw * nw;
   int i[3];
   float w[3][2];
-  for (int d = 0; d < 3; ++d) cic(x[3 * p + d] * scale, i[d], w[d]);
+  for (int d = 0; d < 3; ++d) cic_pos(x, p, d, lt, scale, i[d], w[d]);
   unsigned long long* a = acc + b * n3;
   for (int k3 = 0; k3 < 2; ++k3)
     for (int k2 = 0; k2 < 2; ++k2)
@@ -337,14 +389,14 @@ __global__ void k_cic_deposit(const float* __restrict__ x, int64_t np, int64_t n
       }
 }
 
-__global__ void k_cic_vjp(const float* __restrict__ x, int64_t np, int64_t ntot, int nw, float scale,
+__global__ void k_cic_vjp(const float* __restrict__ x, int64_t np, int64_t ntot, int nw, float scale, Lattice lt,
                           const float* __restrict__ grho, float* __restrict__ gx) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
   const int64_t b = p / np, n3 = (int64_t)nw * nw * nw;
   int i[3];
   float w[3][2];
-  for (int d = 0; d < 3; ++d) cic(x[3 * p + d] * scale, i[d], w[d]);
+  for (int d = 0; d < 3; ++d) cic_pos(x, p, d, lt, scale, i[d], w[d]);
   const float dw[2] = {-1.0f, 1.0f};
   const float* gr = grho + b * n3;
   float o[3] = {0, 0, 0};
@@ -378,16 +430,19 @@ struct cube_pm_ctx {
   PMBuf<unsigned long long> accw;  // B*nw^3 for the CIC density, kept zero
   PMBuf<unsigned int> gmax;
   int nw_alloc = 0;
+  Lattice lat;
   std::string err;
 
   void check_np(int64_t np) const {
     if (np <= 0) throw Error(CUBE_ERR_ARG, "np must be positive");
+    if (lat.L > 0 && np != (int64_t)lat.L * lat.L * lat.L)
+      throw Error(CUBE_ERR_ARG, "with lattice displacements np must be lattice^3");
   }
   // rho -> phi -> F (B*3*n^3) at positions x
   void force_meshes(const float* x, int64_t np, float* F, hipStream_t s) {
     const int n = cfg.n, B = cfg.batch;
     const int64_t ntot = np * B;
-    hipLaunchKernelGGL(k_tsc_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, acc.p);
+    hipLaunchKernelGGL(k_tsc_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, lat, acc.p);
     hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * n3)), dim3(256), 0, s, acc.p, B * n3, 1.0 / kFix, mesh.p);
     fft_convolve<float>(fc, mesh.p, work.p, G.p, B, (float)n3, 0.0f, s);
     hipLaunchKernelGGL(k_fd_force, dim3(nblk(B * n3)), dim3(256), 0, s, mesh.p, B, n, F);
@@ -419,9 +474,12 @@ extern "C" {
 int cube_pm_create(const cube_pm_config* cfg, cube_pm_ctx** out) {
   if (!cfg || !out) return CUBE_ERR_ARG;
   *out = nullptr;
-  if (cfg->n < 8 || cfg->batch < 1 || !(cfg->apm_soft > 0)) return CUBE_ERR_ARG;
+  if (cfg->n < 8 || cfg->batch < 1 || !(cfg->apm_soft > 0) || cfg->lattice < 0 || cfg->lattice > cfg->n)
+    return CUBE_ERR_ARG;
   auto* c = new cube_pm_ctx();
   c->cfg = *cfg;
+  c->lat.L = cfg->lattice;
+  c->lat.step = cfg->lattice > 0 ? (double)cfg->n / (double)cfg->lattice : 0.0;
   const int rc = pm_guarded(c, [&] {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(CUBE_ERR_NO_DEVICE, "no HIP device");
@@ -478,7 +536,7 @@ int cube_pm_force(cube_pm_ctx* ctx, const float* x, int64_t np, float* F, float*
     float* Fm = F ? F : ctx->Fs.p;
     ctx->force_meshes(x, np, Fm, s);
     const int64_t ntot = np * ctx->cfg.batch;
-    hipLaunchKernelGGL(k_tsc_interp, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, ctx->cfg.n, Fm, acc);
+    hipLaunchKernelGGL(k_tsc_interp, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, ctx->cfg.n, ctx->lat, Fm, acc);
     CUBE_HIP(hipGetLastError());
   });
 }
@@ -487,21 +545,25 @@ int cube_pm_force_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, const float*
                       void* stream) {
   return pm_guarded(ctx, [&] {
     ctx->check_np(np);
-    if (!x || !F || !g_acc || !g_x) throw Error(CUBE_ERR_ARG, "null pointer");
+    if (!x || !g_acc || !g_x) throw Error(CUBE_ERR_ARG, "null pointer");
     const hipStream_t s = (hipStream_t)stream;
     const int n = ctx->cfg.n, B = ctx->cfg.batch;
     const int64_t ntot = np * B, n3 = ctx->n3;
+    if (!F) {  // recompute the force meshes of x (checkpoint-and-recompute callers)
+      ctx->force_meshes(x, np, ctx->Fs.p, s);
+      F = ctx->Fs.p;
+    }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctx->gmax.p, 1);
     hipLaunchKernelGGL(k_absmax, dim3(std::min<int64_t>(nblk(3 * ntot), kMaxBlocks)), dim3(256), 0, s, g_acc,
                        3 * ntot, ctx->gmax.p);
-    hipLaunchKernelGGL(k_tsc_deposit_vec, dim3(nblk(ntot)), dim3(256), 0, s, x, g_acc, np, ntot, n, ctx->gmax.p,
-                       ctx->acc.p);
+    hipLaunchKernelGGL(k_tsc_deposit_vec, dim3(nblk(ntot)), dim3(256), 0, s, x, g_acc, np, ntot, n, ctx->lat,
+                       ctx->gmax.p, ctx->acc.p);
     hipLaunchKernelGGL(k_sfix_to_float, dim3(nblk(3 * B * n3)), dim3(256), 0, s, ctx->acc.p, 3 * B * n3, ctx->gmax.p,
                        ctx->Fbar.p);
     hipLaunchKernelGGL(k_fd_adjoint, dim3(nblk(B * n3)), dim3(256), 0, s, ctx->Fbar.p, B, n, ctx->mesh.p);
     fft_convolve<float>(ctx->fc, ctx->mesh.p, ctx->work.p, ctx->G.p, B, (float)n3, 0.0f, s);
-    hipLaunchKernelGGL(k_tsc_vjp_gather, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, F, g_acc, ctx->mesh.p,
-                       g_x);
+    hipLaunchKernelGGL(k_tsc_vjp_gather, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, ctx->lat, F, g_acc,
+                       ctx->mesh.p, g_x);
     CUBE_HIP(hipGetLastError());
   });
 }
@@ -519,7 +581,8 @@ int cube_pm_cic(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, float 
       hipLaunchKernelGGL(k_zero_u64, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3);
       ctx->nw_alloc = nw;
     }
-    hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->accw.p);
+    hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->lat,
+                       ctx->accw.p);
     hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, ctx->accw.p, B * nw3, 1.0 / 2147483648.0,
                        rho);
     CUBE_HIP(hipGetLastError());
@@ -542,7 +605,7 @@ int cube_pm_cic_noalloc(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw
     const int64_t ntot = np * B;
     // the array keeps whatever the last call on this GPU left: clear it here
     hipLaunchKernelGGL(k_zero_u64, dim3(nblk(B * nw3)), dim3(256), 0, s, acc, B * nw3);
-    hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, acc);
+    hipLaunchKernelGGL(k_cic_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->lat, acc);
     hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * nw3)), dim3(256), 0, s, acc, B * nw3, 1.0 / 2147483648.0, rho);
     CUBE_HIP(hipGetLastError());
   });
@@ -555,7 +618,7 @@ int cube_pm_cic_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, int32_t nw, fl
     if (!x || !g_rho || !g_x || nw < 2) throw Error(CUBE_ERR_ARG, "bad argument");
     const hipStream_t s = (hipStream_t)stream;
     const int64_t ntot = np * ctx->cfg.batch;
-    hipLaunchKernelGGL(k_cic_vjp, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, g_rho, g_x);
+    hipLaunchKernelGGL(k_cic_vjp, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, nw, scale, ctx->lat, g_rho, g_x);
     CUBE_HIP(hipGetLastError());
   });
 }

@@ -42,6 +42,7 @@ struct PPScratch {
   int64_t* start = nullptr;   // nbox * m^3 * kSX + 1
   float4* entries = nullptr;  // capacity entries + kPPPad readable padding
   int64_t capacity = 0;
+  unsigned* claim = nullptr;  // 8 work-queue counters of the force kernel
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
 };

@@ -6,10 +6,11 @@
 // atomics, and the in-cell order is the reference's (coarse-cell particle
 // order).
 //
-// Force: one workgroup per brick of PB^3 physical PP cells; each thread owns
-// target particles and sums the pair force from the 27 neighbouring PP cells
-// in a fixed order.  Both members of a physical pair evaluate the pair, so
-// no cross-thread accumulation is needed and the result is deterministic.
+// Force: a persistent kernel whose workgroups claim bricks of PB^3 physical
+// PP cells from a work queue; each thread owns target particles and sums the
+// pair force from the 27 neighbouring PP cells in a fixed order.  Both
+// members of a physical pair evaluate the pair, so no cross-thread
+// accumulation is needed and the result is deterministic.
 // The in-range test uses the reference's FP32 operations on exact FP32
 // positions, so the set of interacting pairs is the reference's.
 #include "pm.h"
@@ -227,9 +228,14 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
   f1[2] = f1[2] + (rv2 / r) * fr;
 }
 
-// One workgroup per brick of PB^3 physical PP cells; each thread owns target
-// particles and sums the pair forces of the 27 neighbouring PP cells in the
-// fixed (dz, dy, x) order.  PP cells are split into kSX sub-bins along x and
+// Persistent: each workgroup claims bricks of PB^3 physical PP cells from one
+// of kPPClaimCounters counters (workgroup i uses counter i % 8, which hands
+// out the bricks 8c + i % 8) until they run out.  The host orders the boxes
+// densest sub-tile first (kick.f90:132-138 sorts by npc_max for
+// schedule(dynamic,1), kick.f90:285), so every counter drains the heaviest
+// bricks first and the launch does not end on a few heavy workgroups.  In a
+// brick each thread owns target particles and sums the pair forces of the
+// 27 neighbouring PP cells in the fixed (dz, dy, x) order.  PP cells are split into kSX sub-bins along x and
 // the CSR is x-ordered along each (y, z) row, so the part of a (dy, dz)
 // column that can hold partners, x in [x_t - r_pp, x_t + r_pp], is one
 // contiguous run found from two sub-bin starts; columns whose y/z distance
@@ -247,136 +253,147 @@ template <typename R>
 constexpr int pp_min_blocks_per_cu() {
   return sizeof(R) == 4 ? 8 : 1;
 }
+constexpr int kPPClaimCounters = 8;
 
 template <typename R, int U>
 __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
-                                                  const int64_t* __restrict__ start,
+                                                  int64_t nbricks, const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
-                                                  PPOut* __restrict__ out) {
+                                                  PPOut* __restrict__ out, unsigned* __restrict__ claim) {
   constexpr int NB = PB + 2, RW = kSX * NB + 1;  // staged sub-bin starts per (y, z) row
   __shared__ int pre[PB * PB * PB + 1];
   __shared__ int64_t rs[NB * NB * RW];
   __shared__ float red[4];
   __shared__ unsigned long long cnt_red[2];
   __shared__ float4 qtab[kQSeg];
+  __shared__ int64_t s_brick;
   if (sizeof(R) == 4) fill_qtab(qtab);
   const int nbr = (L.npgrid + PB - 1) / PB;
-  int64_t b = blockIdx.x;
-  const int bx = (int)(b % nbr);
-  b /= nbr;
-  const int by = (int)(b % nbr);
-  b /= nbr;
-  const int bz = (int)(b % nbr);
-  const int ib = (int)(b / nbr);
-  const int m = L.npgrid + 2;
-  const int64_t mS = (int64_t)m * kSX;
-  const int64_t cbase = (int64_t)ib * m * m * m;
-  const int c0[3] = {1 + bx * PB, 1 + by * PB, 1 + bz * PB};
-  // rows (y, z) = c0-1 .. c0+PB, sub-bins kSX*(c0x-1) .. kSX*(c0x+PB+1); rows
-  // outside the PP grid are empty, sub-bins past the row end read the row end
-  for (int q = threadIdx.x; q < NB * NB * RW; q += blockDim.x) {
-    const int xs = q % RW, y = c0[1] - 1 + (q / RW) % NB, z = c0[2] - 1 + q / (RW * NB);
-    int64_t st = 0;
-    if (y <= L.npgrid + 1 && z <= L.npgrid + 1) {
-      const int64_t row = (cbase + ((int64_t)z * m + y) * m) * kSX;
-      st = start[row + min((int64_t)kSX * (c0[0] - 1) + xs, mS)];
-    }
-    rs[q] = st;
-  }
-  __syncthreads();
+  const int ctr = blockIdx.x % kPPClaimCounters;
   if (threadIdx.x == 0) {
-    int s = 0;
-    for (int q = 0; q < PB * PB * PB; ++q) {
-      pre[q] = s;
-      const int lx = q % PB, ly = (q / PB) % PB, lz = q / (PB * PB);
-      if (c0[0] + lx <= L.npgrid && c0[1] + ly <= L.npgrid && c0[2] + lz <= L.npgrid) {
-        const int row = ((lz + 1) * NB + ly + 1) * RW;
-        s += (int)(rs[row + kSX * (lx + 2)] - rs[row + kSX * (lx + 1)]);
-      }
-    }
-    pre[PB * PB * PB] = s;
     cnt_red[0] = 0;
     cnt_red[1] = 0;
   }
-  __syncthreads();
-  const PPBox B = boxes[ib];
-  const float xoff[3] = {(float)(B.ifs[0] * L.ntt), (float)(B.ifs[1] * L.ntt), (float)(B.ifs[2] * L.ntt)};
-  const int ntg = pre[PB * PB * PB];
   float f2 = 0, vm[3] = {0, 0, 0};
   unsigned long long npp = 0, npg = 0;
-  for (int t = threadIdx.x; t < ntg; t += blockDim.x) {
-    int lo = 0, hi = PB * PB * PB - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (pre[mid] <= t) lo = mid;
-      else hi = mid - 1;
+  for (;;) {
+    __syncthreads();  // the previous brick's tables are no longer read
+    if (threadIdx.x == 0) s_brick = (int64_t)atomicAdd(&claim[ctr], 1u) * kPPClaimCounters + ctr;
+    __syncthreads();
+    int64_t b = s_brick;
+    if (b >= nbricks) break;
+    const int bx = (int)(b % nbr);
+    b /= nbr;
+    const int by = (int)(b % nbr);
+    b /= nbr;
+    const int bz = (int)(b % nbr);
+    const int ib = (int)(b / nbr);
+    const int m = L.npgrid + 2;
+    const int64_t mS = (int64_t)m * kSX;
+    const int64_t cbase = (int64_t)ib * m * m * m;
+    const int c0[3] = {1 + bx * PB, 1 + by * PB, 1 + bz * PB};
+    // rows (y, z) = c0-1 .. c0+PB, sub-bins kSX*(c0x-1) .. kSX*(c0x+PB+1); rows
+    // outside the PP grid are empty, sub-bins past the row end read the row end
+    for (int q = threadIdx.x; q < NB * NB * RW; q += blockDim.x) {
+      const int xs = q % RW, y = c0[1] - 1 + (q / RW) % NB, z = c0[2] - 1 + q / (RW * NB);
+      int64_t st = 0;
+      if (y <= L.npgrid + 1 && z <= L.npgrid + 1) {
+        const int64_t row = (cbase + ((int64_t)z * m + y) * m) * kSX;
+        st = start[row + min((int64_t)kSX * (c0[0] - 1) + xs, mS)];
+      }
+      rs[q] = st;
     }
-    const int lx = lo % PB, ly = (lo / PB) % PB, lz = lo / (PB * PB);
-    const int rowt = ((lz + 1) * NB + ly + 1) * RW;
-    const float4 me = ent[rs[rowt + kSX * (lx + 1)] + (t - pre[lo])];
-    // PP coordinates of the target, the binning's xv = sf (xvec - ifs ntt)
-    // (xvec = xf / rcs exactly for a power-of-two rcs)
-    const float xme[3] = {me.x, me.y, me.z};
-    float fr[3];
-    for (int d = 0; d < 3; ++d) {
-      const float xv = (float)L.sf * (xme[d] * L.inv_rcs - xoff[d]);
-      fr[d] = xv - (float)(c0[d] - 1 + (d == 0 ? lx : (d == 1 ? ly : lz)));  // offset in its PP cell
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int q = 0; q < PB * PB * PB; ++q) {
+        pre[q] = s;
+        const int lx = q % PB, ly = (q / PB) % PB, lz = q / (PB * PB);
+        if (c0[0] + lx <= L.npgrid && c0[1] + ly <= L.npgrid && c0[2] + lz <= L.npgrid) {
+          const int row = ((lz + 1) * NB + ly + 1) * RW;
+          s += (int)(rs[row + kSX * (lx + 2)] - rs[row + kSX * (lx + 1)]);
+        }
+      }
+      pre[PB * PB * PB] = s;
     }
-    // x window in local sub-bins: cells lx .. lx+2 of the staged rows
-    const float u = (fr[0] + (float)(lx + 1)) * (float)kSX;
-    const int wlo = max((int)floorf(u - L.rwin), kSX * lx);
-    const int whi = min((int)floorf(u + L.rwin), kSX * (lx + 3) - 1);
-    R f1[3] = {0, 0, 0};
-    // pairs in range of this target, and those with a ghost partner (32-bit)
-    int npair = 0, nghost = 0;
-    const bool gx0 = c0[0] + lx - 1 == 0, gx1 = c0[0] + lx + 1 == L.npgrid + 1;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy) {
-        const float ey = dy < 0 ? fr[1] : (dy > 0 ? 1.0f - fr[1] : 0.0f);
-        const float ez = dz < 0 ? fr[2] : (dz > 0 ? 1.0f - fr[2] : 0.0f);
-        if (ey * ey + ez * ez >= L.rcut2) continue;
-        const int ny = c0[1] + ly + dy, nz = c0[2] + lz + dz;
-        const bool gyz = ny == 0 || nz == 0 || ny == L.npgrid + 1 || nz == L.npgrid + 1;
-        const int row = rowt + (dz * NB + dy) * RW;
-        const int64_t beg = rs[row + wlo];
-        const int nn = (int)(rs[row + whi + 1] - beg);
-        // candidates [ph0, ph1) of the window are physical: the ghost cells
-        // x-1 / x+1 lie before / after them, a ghost row holds none
-        int ph0 = gx0 ? max((int)(rs[row + kSX * (lx + 1)] - beg), 0) : 0;
-        int ph1 = gx1 ? min((int)(rs[row + kSX * (lx + 2)] - beg), nn) : nn;
-        if (gyz) ph0 = ph1 = 0;
-        const unsigned phn = (unsigned)(ph1 - ph0);
-        const float4* run = ent + beg;
-        for (int p = 0; p < nn; p += U) {
-          float4 o[U];
-#pragma unroll
-          for (int k = 0; k < U; ++k) o[k] = run[p + k];
-#pragma unroll
-          for (int k = 0; k < U; ++k) {
-            const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
-            const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
-            if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
-              pair_force<R>(L, qtab, me, o[k], f1);
-              ++npair;
-              nghost += (unsigned)(p + k - ph0) >= phn;
+    __syncthreads();
+    const PPBox B = boxes[ib];
+    const float xoff[3] = {(float)(B.ifs[0] * L.ntt), (float)(B.ifs[1] * L.ntt), (float)(B.ifs[2] * L.ntt)};
+    const int ntg = pre[PB * PB * PB];
+    for (int t = threadIdx.x; t < ntg; t += blockDim.x) {
+      int lo = 0, hi = PB * PB * PB - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      const int lx = lo % PB, ly = (lo / PB) % PB, lz = lo / (PB * PB);
+      const int rowt = ((lz + 1) * NB + ly + 1) * RW;
+      const float4 me = ent[rs[rowt + kSX * (lx + 1)] + (t - pre[lo])];
+      // PP coordinates of the target, the binning's xv = sf (xvec - ifs ntt)
+      // (xvec = xf / rcs exactly for a power-of-two rcs)
+      const float xme[3] = {me.x, me.y, me.z};
+      float fr[3];
+      for (int d = 0; d < 3; ++d) {
+        const float xv = (float)L.sf * (xme[d] * L.inv_rcs - xoff[d]);
+        fr[d] = xv - (float)(c0[d] - 1 + (d == 0 ? lx : (d == 1 ? ly : lz)));  // offset in its PP cell
+      }
+      // x window in local sub-bins: cells lx .. lx+2 of the staged rows
+      const float u = (fr[0] + (float)(lx + 1)) * (float)kSX;
+      const int wlo = max((int)floorf(u - L.rwin), kSX * lx);
+      const int whi = min((int)floorf(u + L.rwin), kSX * (lx + 3) - 1);
+      R f1[3] = {0, 0, 0};
+      // pairs in range of this target, and those with a ghost partner (32-bit)
+      int npair = 0, nghost = 0;
+      const bool gx0 = c0[0] + lx - 1 == 0, gx1 = c0[0] + lx + 1 == L.npgrid + 1;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const float ey = dy < 0 ? fr[1] : (dy > 0 ? 1.0f - fr[1] : 0.0f);
+          const float ez = dz < 0 ? fr[2] : (dz > 0 ? 1.0f - fr[2] : 0.0f);
+          if (ey * ey + ez * ez >= L.rcut2) continue;
+          const int ny = c0[1] + ly + dy, nz = c0[2] + lz + dz;
+          const bool gyz = ny == 0 || nz == 0 || ny == L.npgrid + 1 || nz == L.npgrid + 1;
+          const int row = rowt + (dz * NB + dy) * RW;
+          const int64_t beg = rs[row + wlo];
+          const int nn = (int)(rs[row + whi + 1] - beg);
+          // candidates [ph0, ph1) of the window are physical: the ghost cells
+          // x-1 / x+1 lie before / after them, a ghost row holds none
+          int ph0 = gx0 ? max((int)(rs[row + kSX * (lx + 1)] - beg), 0) : 0;
+          int ph1 = gx1 ? min((int)(rs[row + kSX * (lx + 2)] - beg), nn) : nn;
+          if (gyz) ph0 = ph1 = 0;
+          const unsigned phn = (unsigned)(ph1 - ph0);
+          const float4* run = ent + beg;
+          for (int p = 0; p < nn; p += U) {
+            float4 o[U];
+  #pragma unroll
+            for (int k = 0; k < U; ++k) o[k] = run[p + k];
+  #pragma unroll
+            for (int k = 0; k < U; ++k) {
+              const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
+              const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
+              if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
+                pair_force<R>(L, qtab, me, o[k], f1);
+                ++npair;
+                nghost += (unsigned)(p + k - ph0) >= phn;
+              }
             }
           }
         }
+      npg += (unsigned)nghost;
+      npp += (unsigned)(npair - nghost);
+      // update v (kick.f90:576-582): g = fl32(f mass_scale) as the reference
+      // forms it, times the FP64 coefficient, added in FP64
+      const unsigned ip = __float_as_uint(me.w);
+      float g[3];
+      for (int d = 0; d < 3; ++d) g[d] = (float)f1[d] * L.mass_scale;
+      f2 = fmaxf(f2, (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+      for (int d = 0; d < 3; ++d) {
+        const float v = (float)((double)vp[3 * (int64_t)ip + d] + (double)g[d] * L.v_scale);
+        vp[3 * (int64_t)ip + d] = v;
+        vm[d] = fmaxf(vm[d], fabsf(v));
       }
-    npg += (unsigned)nghost;
-    npp += (unsigned)(npair - nghost);
-    // update v (kick.f90:576-582): g = fl32(f mass_scale) as the reference
-    // forms it, times the FP64 coefficient, added in FP64
-    const unsigned ip = __float_as_uint(me.w);
-    float g[3];
-    for (int d = 0; d < 3; ++d) g[d] = (float)f1[d] * L.mass_scale;
-    f2 = fmaxf(f2, (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
-    for (int d = 0; d < 3; ++d) {
-      const float v = (float)((double)vp[3 * (int64_t)ip + d] + (double)g[d] * L.v_scale);
-      vp[3 * (int64_t)ip + d] = v;
-      vm[d] = fmaxf(vm[d], fabsf(v));
     }
-  }
+  }  // claimed bricks
   auto bmax = [&](float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
     __syncthreads();
@@ -447,16 +464,33 @@ void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, cons
 
 void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, float* vp, PPScratch& S, PPOut* out,
               hipStream_t s, int unroll) {
-  const int nbr = (L.npgrid + PB - 1) / PB;
-  const dim3 grid((unsigned)((int64_t)nbr * nbr * nbr * nbox));
+  const int64_t nbr = (L.npgrid + PB - 1) / PB;
+  const int64_t nbricks = nbr * nbr * nbr * nbox;
+  static int n_cu = 0;
+  if (n_cu == 0) {
+    int dev = 0;
+    CUBE_HIP(hipGetDevice(&dev));
+    CUBE_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  zero_async(S.claim, sizeof(unsigned) * kPPClaimCounters, s);
+  // as many workgroups as fit on the chip at once (eight per CU in the FP32
+  // form, the launch bound's occupancy; four in the FP64 form, whose 117
+  // VGPRs allow four wavefronts per SIMD), each then claims bricks until
+  // none remain
+  const int per_cu = precision == 64 ? 4 : pp_min_blocks_per_cu<float>();
+  const dim3 grid((unsigned)std::min<int64_t>(nbricks, (int64_t)n_cu * per_cu));
   if (precision == 64)
-    hipLaunchKernelGGL((k_pp_force<double, 6>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
+    hipLaunchKernelGGL((k_pp_force<double, 6>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
+                       out, S.claim);
   else if (unroll == 4)
-    hipLaunchKernelGGL((k_pp_force<float, 4>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
+    hipLaunchKernelGGL((k_pp_force<float, 4>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
+                       out, S.claim);
   else if (unroll == 8)
-    hipLaunchKernelGGL((k_pp_force<float, 8>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
+    hipLaunchKernelGGL((k_pp_force<float, 8>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
+                       out, S.claim);
   else
-    hipLaunchKernelGGL((k_pp_force<float, 6>), grid, dim3(256), 0, s, L, boxes, S.start, S.entries, vp, out);
+    hipLaunchKernelGGL((k_pp_force<float, 6>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
+                       out, S.claim);
   CUBE_HIP(hipGetLastError());
 }
 

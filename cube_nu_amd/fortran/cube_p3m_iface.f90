@@ -26,6 +26,7 @@ module cube_p3m_iface
     real(c_double) :: f_nu, mass_nu
     real(c_float) :: image_buffer, tile_buffer
     integer(c_int64_t) :: workspace_bytes
+    integer(c_int32_t) :: ires_fixed   ! 0: adaptive sub-tile resolution; 2..5: fixed
   end type
 
   type, bind(C) :: cube_kick_in
@@ -217,6 +218,6 @@ contains
     cfg%ng = ng; cfg%nnt = 4; cfg%nns = 4; cfg%ratio_cs = 4; cfg%ncb = 4; cfg%ngb = 16; cfg%izipx = 2
     cfg%precision = 32; cfg%box = box; cfg%apm1c = 3.5; cfg%apm2 = 3.5; cfg%apm3f = 3.5; cfg%app = 0.06
     cfg%pm3 = 1; cfg%pp = 1; cfg%pp_corr = 1; cfg%f_nu = 0; cfg%mass_nu = 0
-    cfg%image_buffer = 2.0; cfg%tile_buffer = 6.0; cfg%workspace_bytes = 0
+    cfg%image_buffer = 2.0; cfg%tile_buffer = 6.0; cfg%workspace_bytes = 0; cfg%ires_fixed = 0
   end function
 end module cube_p3m_iface

@@ -56,7 +56,7 @@ class Config(C.Structure):
         ("box", C.c_float), ("apm1c", C.c_float), ("apm2", C.c_float), ("apm3f", C.c_float),
         ("app", C.c_float), ("pm3", C.c_int32), ("pp", C.c_int32), ("pp_corr", C.c_int32),
         ("f_nu", C.c_double), ("mass_nu", C.c_double), ("image_buffer", C.c_float),
-        ("tile_buffer", C.c_float), ("workspace_bytes", C.c_int64),
+        ("tile_buffer", C.c_float), ("workspace_bytes", C.c_int64), ("ires_fixed", C.c_int32),
     ]
 
 
@@ -242,11 +242,12 @@ class P3M:
                  ratio_cs: int = 4, ncb: int = 4, ngb: int = 16, box: float = 600.0, apm1c: float = 3.5,
                  apm2: float = 3.5, apm3f: float = 3.5, app: float = 0.06, pm3: bool = True, pp: bool = True,
                  pp_corr: bool = True, f_nu: float = 0.0, mass_nu: float = 0.0, image_buffer: float = 2.0,
-                 tile_buffer: float = 6.0, workspace_bytes: int = 0):
+                 tile_buffer: float = 6.0, workspace_bytes: int = 0, ires_fixed: int = 0):
         self.lib = load_library()
         self.geo = Geometry(ng, nnt, nns, ratio_cs, ncb, ngb, izipx, image_buffer, tile_buffer)
         cfg = Config(ng, nnt, nns, ratio_cs, ncb, ngb, izipx, precision, box, apm1c, apm2, apm3f, app, int(pm3),
-                     int(pp), int(pp_corr), f_nu, mass_nu, image_buffer, tile_buffer, workspace_bytes)
+                     int(pp), int(pp_corr), f_nu, mass_nu, image_buffer, tile_buffer, workspace_bytes,
+                     int(ires_fixed))
         self.cfg = cfg
         ctx = C.c_void_p()
         torch_runtime_first()

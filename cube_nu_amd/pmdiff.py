@@ -11,12 +11,17 @@ particle_mesh.f90:36-205, Green.f90:22-71 with apm_range = 0):
   stream; the arithmetic is the library's).
 * ``pm_force`` / ``cic_density`` -- ``torch.autograd.Function`` wrappers whose
   backward calls the hand-written adjoint kernels.
-* ``kdk`` -- a multi-step kick-drift-kick PM forward built from them; torch's
-  autograd keeps each step's positions and force meshes (per-step
-  checkpoints), so the backward pass replays the adjoint step by step without
-  recomputing the forward.
+* ``kdk`` -- a multi-step kick-drift-kick PM forward as one
+  ``torch.autograd.Function``: the forward keeps only each step's particle
+  positions (per-step checkpoints of the particle state); the backward walks
+  the steps in reverse, recomputing each force mesh from the checkpointed
+  positions before its adjoint, so no mesh outlives its step.
 * ``zeldovich`` -- a differentiable (torch) Zel'dovich initial condition from
-  white-noise modes, so gradients reach the noise.
+  white-noise modes, so gradients reach the noise.  With ``displacement=True``
+  it returns the displacements from the particle lattice, the form a context
+  created with ``lattice=ng`` takes (its floor indices and fractions then
+  carry the rounding of the displacement rather than of the absolute
+  coordinate).
 
 No CPU fallback: every function raises if the library or a GPU is missing.
 """
@@ -37,7 +42,7 @@ PM_ABI_FUNCTIONS = (
 
 
 class PMConfig(C.Structure):
-    _fields_ = [("n", C.c_int32), ("batch", C.c_int32), ("apm_soft", C.c_float)]
+    _fields_ = [("n", C.c_int32), ("batch", C.c_int32), ("apm_soft", C.c_float), ("lattice", C.c_int32)]
 
 
 def _lib():
@@ -72,13 +77,17 @@ def _stream() -> C.c_void_p:
 class PMForce:
     """PM force on ``batch`` periodic meshes of n^3 points (mesh units)."""
 
-    def __init__(self, n: int, batch: int = 1, apm_soft: float = 1.0):
+    def __init__(self, n: int, batch: int = 1, apm_soft: float = 1.0, lattice: int = 0):
+        """lattice = L > 0: positions are displacements from an L^3 lattice
+        of sites ((i + 1/2) n / L per axis), one per particle in x-fastest
+        order (include/cube_pm.h)."""
         if not torch.cuda.is_available():
             raise RuntimeError("PMForce needs a GPU (the HIP library is the only implementation)")
         self.lib = _lib()
-        self.n, self.batch = int(n), int(batch)
+        self.n, self.batch, self.lattice = int(n), int(batch), int(lattice)
         self.ctx = C.c_void_p()
-        rc = self.lib.cube_pm_create(C.byref(PMConfig(self.n, self.batch, float(apm_soft))), C.byref(self.ctx))
+        rc = self.lib.cube_pm_create(C.byref(PMConfig(self.n, self.batch, float(apm_soft), self.lattice)),
+                                     C.byref(self.ctx))
         if rc != 0:
             raise CubeError(rc, f"cube_pm_create failed ({rc})")
 
@@ -121,11 +130,21 @@ class PMForce:
         self._check(self.lib.cube_pm_force(self.ctx, _ptr(x), np_, _ptr(F), _ptr(acc), _stream()))
         return acc, F
 
-    def force_vjp(self, x: torch.Tensor, F: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+    def force_vjp(self, x: torch.Tensor, F: torch.Tensor | None, g: torch.Tensor) -> torch.Tensor:
+        """(d acc / d x)^T g; F = None recomputes the force meshes of x."""
         gx = torch.empty_like(x)
-        self._check(self.lib.cube_pm_force_vjp(self.ctx, _ptr(x), self._np(x), _ptr(F), _ptr(g.contiguous()),
-                                               _ptr(gx), _stream()))
+        self._check(self.lib.cube_pm_force_vjp(self.ctx, _ptr(x), self._np(x), None if F is None else _ptr(F),
+                                               _ptr(g.contiguous()), _ptr(gx), _stream()))
         return gx
+
+    def sites(self, dtype=torch.float64, device="cpu") -> torch.Tensor:
+        """Lattice site positions (np, 3) in mesh units, x fastest (lattice mode)."""
+        L = self.lattice
+        if L <= 0:
+            raise ValueError("not a lattice-displacement context")
+        g = (torch.arange(L, dtype=dtype, device=device) + 0.5) * (self.n / L)
+        QZ, QY, QX = torch.meshgrid(g, g, g, indexing="ij")
+        return torch.stack([QX, QY, QZ], dim=-1).reshape(L ** 3, 3)
 
     def cic(self, x: torch.Tensor, nw: int, scale: float) -> torch.Tensor:
         rho = torch.empty((self.batch, nw, nw, nw), dtype=torch.float32, device=x.device)
@@ -178,16 +197,59 @@ def cic_density(x: torch.Tensor, pm: PMForce, nw: int, scale: float) -> torch.Te
     return _CICFn.apply(x, pm, nw, scale)
 
 
+class _KDKFn(torch.autograd.Function):
+    """Kick-drift-kick steps with per-step checkpoints of the positions.
+
+    Forward: a_0 = F(x_0); for each step s: v' = v_s + k1_s a_s,
+    x_{s+1} = x_s + d_s v', a_{s+1} = F(x_{s+1}), v_{s+1} = v' + k2_s a_{s+1}.
+    Only x_0 .. x_S are kept.  Backward, with the cotangents (X, V) of
+    (x_{s+1}, v_{s+1}) and A the part of a_{s+1}'s cotangent that step s+1
+    left (k1_{s+1} V' of that step): A += k2_s V; X += J_F(x_{s+1})^T A
+    (the force mesh of x_{s+1} recomputed first); V' = V + d_s X; then
+    (X, V) become the cotangents of (x_s, v_s) and A = k1_s V'.  Finally
+    X += J_F(x_0)^T A.
+    """
+
+    @staticmethod
+    def forward(ctx, x, v, pm, kick1, drift, kick2):
+        x = x.contiguous()
+        keep = [x]
+        a, _ = pm.force(x)
+        for k1, d, k2 in zip(kick1, drift, kick2):
+            v = v + k1 * a
+            x = (x + d * v).contiguous()
+            keep.append(x)
+            a, _ = pm.force(x)
+            v = v + k2 * a
+        ctx.pm, ctx.coef = pm, (list(kick1), list(drift), list(kick2))
+        ctx.save_for_backward(*keep)
+        return x, v
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        xs = ctx.saved_tensors
+        kick1, drift, kick2 = ctx.coef
+        pm = ctx.pm
+        X = torch.zeros_like(xs[-1]) if gx is None else gx.clone()
+        V = torch.zeros_like(xs[-1]) if gv is None else gv.clone()
+        A = None
+        for s in range(len(drift) - 1, -1, -1):
+            A = kick2[s] * V if A is None else A + kick2[s] * V
+            X = X + pm.force_vjp(xs[s + 1], None, A)
+            V = V + drift[s] * X
+            A = kick1[s] * V
+        if A is not None:
+            X = X + pm.force_vjp(xs[0], None, A)
+        return X, V, None, None, None, None
+
+
 def kdk(x: torch.Tensor, v: torch.Tensor, pm: PMForce, kick1, drift, kick2):
     """Kick-drift-kick steps v += k1 a(x); x += d v; v += k2 a(x), with the
-    force at the end of a step reused by the next step's first kick."""
-    a = pm_force(x, pm)
-    for k1, d, k2 in zip(kick1, drift, kick2):
-        v = v + k1 * a
-        x = x + d * v
-        a = pm_force(x, pm)
-        v = v + k2 * a
-    return x, v
+    force at the end of a step reused by the next step's first kick.
+    Differentiable with respect to x and v; the backward recomputes each
+    step's force mesh from checkpointed positions (memory: one position array
+    per step, no meshes)."""
+    return _KDKFn.apply(x, v, pm, list(kick1), list(drift), list(kick2))
 
 
 def kdk_coefficients(a0: float, a1: float, nsteps: int, omega_m: float = 0.301):
@@ -215,11 +277,13 @@ def bbks_amplitude(k, h=0.6817, omega_m=0.301, n_s=0.9693):
     return torch.sqrt(k ** n_s) * T
 
 
-def zeldovich(noise: torch.Tensor, box: float, n_mesh: int, amp: float):
+def zeldovich(noise: torch.Tensor, box: float, n_mesh: int, amp: float, displacement: bool = False):
     """Positions (mesh units) and velocities of ng^3 particles per batch
     from white noise (batch, ng, ng, ng): delta_k = amp * sqrt(P(k)) *
     FFT(noise), displacement psi_k = i k/k^2 delta_k, x = q + psi, v = psi
-    (growing mode, in units where the drift factor carries the time)."""
+    (growing mode, in units where the drift factor carries the time).
+    displacement=True returns psi (mesh units) instead of x, for a context
+    with lattice=ng (the sites q are (i + 1/2) n_mesh / ng, x fastest)."""
     B, ng = noise.shape[0], noise.shape[1]
     dk = torch.fft.rfftn(noise, dim=(1, 2, 3))
     kf = 2 * math.pi / box
@@ -240,6 +304,8 @@ def zeldovich(noise: torch.Tensor, box: float, n_mesh: int, amp: float):
     QZ, QY, QX = torch.meshgrid(g, g, g, indexing="ij")
     q = torch.stack([QX, QY, QZ], dim=-1).expand(B, ng, ng, ng, 3)
     s = n_mesh / ng
-    x = ((q + psi) * s).reshape(B * ng ** 3, 3).contiguous()
     v = (psi * s).reshape(B * ng ** 3, 3).contiguous()
+    if displacement:
+        return v.clone(), v
+    x = ((q + psi) * s).reshape(B * ng ** 3, 3).contiguous()
     return x, v

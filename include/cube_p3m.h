@@ -69,6 +69,9 @@ typedef struct cube_p3m_config {
   float image_buffer;  /* parameters.f90:85 */
   float tile_buffer;   /* parameters.f90:86 */
   int64_t workspace_bytes; /* device budget for batched tile meshes; 0 = auto */
+  int32_t ires_fixed;  /* 0: sub-tile PM3/PP resolution from the occupancy
+                          thresholds (kick.f90:116-130); 2..5: every sub-tile
+                          at that ires (the commented ires(itile)=4, kick.f90:131) */
 } cube_p3m_config;
 
 /* Kick inputs: the module scalars kick.f90 reads (variables.f90:35, sim). */

@@ -42,6 +42,14 @@ typedef struct cube_pm_config {
   int32_t n;        /* mesh points per dimension (a size the FFT supports)   */
   int32_t batch;    /* B independent meshes                                  */
   float apm_soft;   /* Green_3D softening in mesh cells (> 0)                */
+  int32_t lattice;  /* 0: x holds absolute positions.  L > 0: x holds
+                       displacements from a lattice of L^3 sites per mesh:
+                       particle p of a batch (np = L^3) sits at site
+                       p mod L^3, x fastest, at ((i + 1/2) n / L) per axis,
+                       plus x.  Floor indices and fractions then carry the
+                       rounding of the displacement, not of the absolute
+                       coordinate (up to n cells), and gradients with respect
+                       to x are those with respect to the positions.         */
 } cube_pm_config;
 
 typedef struct cube_pm_ctx cube_pm_ctx;
@@ -61,7 +69,9 @@ int cube_pm_force(cube_pm_ctx* ctx, const float* x, int64_t np, float* F, float*
 
 /* g_x = (d acc / d x)^T g_acc, through both the interpolation weights and
  * the meshes (deposit -> convolution -> FD).  F is the force mesh of the
- * same x.  g_x is overwritten. */
+ * same x, or NULL: the force meshes of x are then recomputed into context
+ * scratch first (callers that checkpoint positions only).  g_x is
+ * overwritten. */
 int cube_pm_force_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, const float* F, const float* g_acc, float* g_x,
                       void* stream);
 

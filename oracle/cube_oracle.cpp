@@ -45,6 +45,7 @@ struct OParams {
   int32_t pm3, pp, pp_corr;
   double f_nu, mass_nu;
   float tile_buffer, image_buffer;
+  int32_t ires_fixed;  // 0: thresholds of kick.f90:116-130; 2..5: fixed (kick.f90:131, commented there)
 };
 
 struct OKickIn {
@@ -857,6 +858,24 @@ int orc_poisson(int n, const float* rho, const float* tf, float* phi) {
   return 0;
 }
 
+// Timing sample for the CPU baseline: with a stride s > 1 the kick's PM2, PM3
+// and PP loops and the drift process only tiles T with T % s == 0 (PM1, a
+// global mesh, runs whole); the outputs are then incomplete and serve only to
+// time the work.  Seconds per phase of the last kick/drift, in the reference's
+// tcat slots: [0] PM1 (11), [1] PM2 (12), [2] PM3 (13), [3] PP (14), [4] drift (2).
+static int64_t g_sample_stride = 1;
+static double g_phase_s[5] = {0, 0, 0, 0, 0};
+
+int orc_set_sample_stride(int64_t s) {
+  g_sample_stride = s < 1 ? 1 : s;
+  return 0;
+}
+
+int orc_phase_seconds(double* t) {
+  for (int i = 0; i < 5; ++i) t[i] = g_phase_s[i];
+  return 0;
+}
+
 // Classification and napm (kick.f90:108-133).
 int orc_classify(const OParams* P, const int32_t* rhoc, int32_t* ires, int32_t napm[7]) {
   Geo g(*P);
@@ -865,7 +884,8 @@ int orc_classify(const OParams* P, const int32_t* rhoc, int32_t* ires, int32_t n
   for (int64_t it = 0; it < ns3; ++it) {
     int64_t T, ifs[3];
     subtile_ids(g, it, &T, ifs);
-    ires[it] = classify(g, rhoc, T, ifs);
+    // a fixed level replaces the thresholds (the commented ires(itile)=4, kick.f90:131)
+    ires[it] = P->ires_fixed >= 2 ? P->ires_fixed : classify(g, rhoc, T, ifs);
     napm[ires[it] - 1]++;
   }
   return 0;
@@ -880,6 +900,9 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
   std::vector<int64_t> off = cell_offsets(g, rhoc);
   int mask = kin->phase_mask ? kin->phase_mask : 15;
   float vmax[3] = {0, 0, 0};
+  const int64_t stride = g_sample_stride;
+  for (int i = 0; i < 4; ++i) g_phase_s[i] = 0;
+  double t0 = omp_get_wtime();
   if (mask & 1) {
     OKickOut o1;
     std::memset(&o1, 0, sizeof(o1));
@@ -887,11 +910,14 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
     out->dt_pm1 = o1.dt_pm1; out->f2max_pm1 = o1.f2max_pm1; out->testrho = o1.testrho;
     for (int d = 0; d < 3; ++d) vmax[d] = o1.vmax[d];
   }
+  g_phase_s[0] = omp_get_wtime() - t0;
+  t0 = omp_get_wtime();
   float gg = g_grid();
   if (mask & 2) {
     float f2 = 0, vm[3] = {0, 0, 0};
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t T = 0; T < g.ntiles(); ++T) {
+      if (T % stride) continue;
       float f2t, vt[3];
       LevelBox b = pm2_box(g, T);
       pm_box(g, xp, vp, rhoc, off, tf[1], b, *kin, &f2t, vt);
@@ -902,6 +928,8 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
     out->dt_pm2 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2) * (float)kin->a_mid) * gg));
     for (int d = 0; d < 3; ++d) vmax[d] = vm[d];
   }
+  g_phase_s[1] = omp_get_wtime() - t0;
+  t0 = omp_get_wtime();
   int64_t ns3 = g.ntiles() * g.nns * g.nns * g.nns;
   std::vector<int32_t> ires(ns3);
   if (mask & 12) orc_classify(P, rhoc, ires.data(), out->napm);
@@ -911,6 +939,7 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
     for (int64_t it = 0; it < ns3; ++it) {
       int64_t T, ifs[3];
       subtile_ids(g, it, &T, ifs);
+      if (T % stride) continue;
       float f2t, vt[3];
       LevelBox b = pm3_box(g, T, ifs, ires[it]);
       pm_box(g, xp, vp, rhoc, off, tf[ires[it]], b, *kin, &f2t, vt);
@@ -921,6 +950,8 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
     out->dt_pm3 = 0.5f * std::sqrt(1.0f / ((std::sqrt(f2) * (float)kin->a_mid) * gg));
     for (int d = 0; d < 3; ++d) vmax[d] = vm[d];
   }
+  g_phase_s[2] = omp_get_wtime() - t0;
+  t0 = omp_get_wtime();
   int64_t totpairs = 0;
   if ((mask & 8) && P->pp) {
     float f2 = 0, vm[3] = {0, 0, 0};
@@ -928,6 +959,7 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
     for (int64_t it = 0; it < ns3; ++it) {
       int64_t T, ifs[3];
       subtile_ids(g, it, &T, ifs);
+      if (T % stride) continue;
       float f2t = 0, vt[3] = {0, 0, 0};
       int64_t np = 0;
       pp_box(g, *P, xp, vp, rhoc, off, T, ifs, ires[it], *kin, &f2t, vt, &np);
@@ -939,6 +971,7 @@ int orc_kick(const OParams* P, const void* xp, float* vp, const int32_t* rhoc, c
     out->dt_pp = 0.05f * std::sqrt(1.0f / ((std::sqrt(f2) * (float)kin->a_mid) * gg));
     for (int d = 0; d < 3; ++d) vmax[d] = vm[d];
   }
+  g_phase_s[3] = omp_get_wtime() - t0;
   if (npairs_out) *npairs_out = totpairs;
   for (int d = 0; d < 3; ++d) out->vmax[d] = vmax[d];
   float mv = std::max({vmax[0], vmax[1], vmax[2]});
@@ -1046,7 +1079,9 @@ int orc_drift(const OParams* P, const void* xp, const float* vp, const int64_t* 
   std::vector<int32_t> rhoce(ne * ne * ne), rholocal(ne * ne * ne);
   std::vector<int64_t> xnew, pnew;
   std::vector<float> vnew;
+  const double t0 = omp_get_wtime();
   for (int64_t T = 0; T < g.ntiles(); ++T) {
+    if (T % g_sample_stride) continue;
     std::fill(rhoce.begin(), rhoce.end(), 0);
     std::fill(rholocal.begin(), rholocal.end(), 0);
     auto dest = [&](int64_t ip, const int64_t ijk[3], int64_t gi[3]) -> bool {
@@ -1132,6 +1167,7 @@ int orc_drift(const OParams* P, const void* xp, const float* vp, const int64_t* 
           rhoc_out[g.rc(i, j, k, T)] = rhoce[e];
         }
   }
+  g_phase_s[4] = omp_get_wtime() - t0;
   out->nplocal = np_prev;
   out->overhead_tile = overhead;
   if (pid_out)

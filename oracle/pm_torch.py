@@ -1,8 +1,10 @@
 """TEST INFRASTRUCTURE ONLY -- plain-PyTorch FP64 restatement of the
 differentiable PM operator of include/cube_pm.h, for the checker.
 
-Only tests/ and bench.py's checks may import it.  It evaluates on the CPU in
-float64, with torch's autograd providing the exact gradient of the same
+Only tests/ and bench.py's checks may import it.  It evaluates in float64 on
+the device of its inputs (the CPU for the small cases; torch's own GPU
+kernels for the 256^3-mesh comparison, which would take many minutes on the
+CPU), with torch's autograd providing the exact gradient of the same
 discrete model, against which the HIP kernels' values and hand-written
 adjoints are compared.  The reference has no PM-only differentiable path;
 the pieces restated are PM1's (TSC deposit and interpolation,
@@ -42,8 +44,8 @@ def deposit(x: torch.Tensor, batch: int, n: int) -> torch.Tensor:
     """rho (batch, n, n, n) [z][y][x] = sum_p W(x_p - m)."""
     np_ = x.shape[0] // batch
     i, w = tsc_weights(x)
-    b = torch.arange(batch).repeat_interleave(np_)
-    rho = torch.zeros(batch * n ** 3, dtype=x.dtype)
+    b = torch.arange(batch, device=x.device).repeat_interleave(np_)
+    rho = torch.zeros(batch * n ** 3, dtype=x.dtype, device=x.device)
     for k3 in range(3):
         for k2 in range(3):
             for k1 in range(3):
@@ -72,7 +74,7 @@ def interp(x: torch.Tensor, F: torch.Tensor) -> torch.Tensor:
     batch, n = F.shape[0], F.shape[-1]
     np_ = x.shape[0] // batch
     i, w = tsc_weights(x)
-    b = torch.arange(batch).repeat_interleave(np_)
+    b = torch.arange(batch, device=x.device).repeat_interleave(np_)
     Ff = F.permute(0, 2, 3, 4, 1).reshape(batch * n ** 3, 3)
     acc = torch.zeros_like(x)
     for k3 in range(3):
@@ -94,8 +96,8 @@ def cic(x: torch.Tensor, batch: int, nw: int, scale: float) -> torch.Tensor:
     d = y - fi
     i = fi.long()
     w = torch.stack([1 - d, d], dim=-1)
-    b = torch.arange(batch).repeat_interleave(np_)
-    rho = torch.zeros(batch * nw ** 3, dtype=x.dtype)
+    b = torch.arange(batch, device=x.device).repeat_interleave(np_)
+    rho = torch.zeros(batch * nw ** 3, dtype=x.dtype, device=x.device)
     for k3 in range(2):
         for k2 in range(2):
             for k1 in range(2):

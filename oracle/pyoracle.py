@@ -22,6 +22,7 @@ class OParams(C.Structure):
         ("ngb", C.c_int32), ("izipx", C.c_int32), ("app", C.c_float), ("apm1c", C.c_float), ("apm2", C.c_float),
         ("apm3f", C.c_float), ("pm3", C.c_int32), ("pp", C.c_int32), ("pp_corr", C.c_int32), ("f_nu", C.c_double),
         ("mass_nu", C.c_double), ("tile_buffer", C.c_float), ("image_buffer", C.c_float),
+        ("ires_fixed", C.c_int32),
     ]
 
 
@@ -69,9 +70,9 @@ def _p(a):
 
 
 def params(ng, izipx=2, nnt=4, nns=4, ratio_cs=4, ncb=4, ngb=16, app=0.06, apm1c=3.5, apm2=3.5, apm3f=3.5, pm3=1,
-           pp=1, pp_corr=1, f_nu=0.0, mass_nu=0.0, tile_buffer=6.0, image_buffer=2.0):
+           pp=1, pp_corr=1, f_nu=0.0, mass_nu=0.0, tile_buffer=6.0, image_buffer=2.0, ires_fixed=0):
     return OParams(ng, nnt, nns, ratio_cs, ncb, ngb, izipx, app, apm1c, apm2, apm3f, pm3, pp, pp_corr, f_nu,
-                   mass_nu, tile_buffer, image_buffer)
+                   mass_nu, tile_buffer, image_buffer, ires_fixed)
 
 
 def green(n, apm_soft, apm_range, gridsize):
@@ -253,3 +254,17 @@ def cicpower(P, xp, rhoc_phys, nw, box, want_delta=False):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_sample_stride(stride):
+    """Time-only sample: kick PM2/PM3/PP and drift process tiles T with
+    T % stride == 0 (1 = every tile, the default)."""
+    lib().orc_set_sample_stride(C.c_int64(int(stride)))
+
+
+def phase_seconds():
+    """Seconds of the last kick's PM1, PM2, PM3, PP phases and the last drift
+    (the reference's tcat slots 11, 12, 13, 14 and 2)."""
+    t = np.zeros(5, np.float64)
+    lib().orc_phase_seconds(_p(t))
+    return t

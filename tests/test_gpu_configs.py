@@ -1,0 +1,239 @@
+"""GPU parity against the oracle at the headline workloads.
+
+cfg2: 512^3 CDM in 600 Mpc/h, the bench's seeded Zel'dovich input at z=2
+(int16 and int8 coordinates).  Every kick phase (PM1, PM2, PM3, PP: PM1's
+128^3 mesh, PM2's 64 tile meshes of 160^3 and PM3's 4096 sub-tile meshes of
+80^3, the FFT sizes the step runs) and the full kick against the oracle's
+kick.f90 restatement (rms relative error of the velocity change <= 1e-5,
+f2max and the Delta-t criteria to 1e-5, napm exact), then one drift after a
+resident kick + buffer_v, bit-exact in rhoc, order, re-encoded coordinates,
+velocities and particle IDs (drift.f90:1-233).
+
+cfg1: 128^3 CDM in 200 Mpc/h, int8 coordinates, Zel'dovich at z=49, PM3 at
+ratio_sf = 4 for every sub-tile (ires = 3), PP on, 10 steps of a fixed code
+time step to z=10 in the main loop's order (drift, buffer, kick, buffer_v;
+main.f90:30-38), then the closing half drift (main.f90:39-41).  Every step is
+checked against the oracle run on the GPU's own pre-step state (drift and
+buffer bit-exact, kick rms <= 1e-5), and an independent oracle run of the
+same 10 steps is compared with the GPU's by particle ID at the end.
+"""
+import numpy as np
+import pytest
+
+from cube_nu_amd import P3M, cosmology
+from cube_nu_amd.p3m import Geometry
+from inputs import oparams, physical_mask, rel_rms
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20260101
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _cap(geo):
+    return int(geo.ng ** 3 * (geo.nte / geo.nt) ** 3 * 1.15) + 4096
+
+
+class State512:
+    def __init__(self, izipx):
+        ng, z, box = 512, 2.0, 600.0
+        self.geo = Geometry(ng, izipx=izipx)
+        self.p = P3M(ng, izipx=izipx, box=box)
+        self.a = 1.0 / (1.0 + z)
+        ic = self.p.make_ic(seed=SEED, z=z, nic=2)
+        k0 = self.p.kick_resident(a_mid=self.a, dt=0.0)
+        self.dt = min(k0["dt_pm1"], k0["dt_pm2"], k0["dt_pm3"], k0["dt_pp"], 1.0)
+        self.xe, self.ve, self.pe, self.re = self.p.download(cap=_cap(self.geo))
+        self.tfs = orc.tf_default(oparams(self.geo))
+        _log(f"512^3 izipx {izipx}: {self.xe.shape[0]} extended particles, dt {self.dt:.5g}, "
+             f"vz_max {ic['vz_max']:.3f}, oracle threads {orc.num_threads()}")
+
+
+@pytest.fixture(scope="module")
+def s512():
+    s = State512(2)
+    yield s
+    s.p.close()
+
+
+@pytest.mark.parametrize("mask,key", [(1, "f2max_pm1"), (2, "f2max_pm2"), (4, "f2max_pm3"), (8, "f2max_pp")])
+def test_cfg2_512_kick_phase(s512, mask, key):
+    """One phase from zero velocities: the velocity change is the force times
+    the kick factor a_mid dt / 6 pi."""
+    v_g = np.zeros_like(s512.ve)
+    out_g = s512.p.kick(s512.xe, v_g, s512.re, a_mid=s512.a, dt=s512.dt, phase_mask=mask)
+    v_o = np.zeros_like(s512.ve)
+    out_o, _, npairs = orc.kick(oparams(s512.geo), s512.xe, v_o, s512.re, s512.tfs, s512.a, s512.dt,
+                                phase_mask=mask)
+    err = rel_rms(v_g, v_o)
+    _log(f"512^3 phase {mask}: rms rel dv error {err:.3e}, {key} {out_g[key]:.6g} vs {out_o[key]:.6g}"
+         + (f", napm {out_g['napm']}" if mask & 12 else "") + (f", pairs {npairs}" if mask == 8 else ""))
+    assert err <= 1e-5
+    assert abs(out_g[key] - out_o[key]) <= 1e-5 * abs(out_o[key])
+    if mask & 12:
+        assert out_g["napm"] == out_o["napm"]
+    if mask == 8:
+        assert s512.p.timing_report()["pp_pairs_last"]["calls"] == npairs
+
+
+def test_cfg2_512_full_kick(s512):
+    v_g = s512.ve.copy()
+    out_g = s512.p.kick(s512.xe, v_g, s512.re, a_mid=s512.a, dt=s512.dt)
+    v_o = s512.ve.copy()
+    out_o, _, _ = orc.kick(oparams(s512.geo), s512.xe, v_o, s512.re, s512.tfs, s512.a, s512.dt)
+    err = rel_rms(v_g - s512.ve, v_o - s512.ve)
+    _log(f"512^3 full kick: rms rel dv error {err:.3e}")
+    assert err <= 1e-5
+    for k in ("dt_pm1", "dt_pm2", "dt_pm3", "dt_pp", "dt_vmax", "vz_max"):
+        assert abs(out_g[k] - out_o[k]) <= 1e-5 * abs(out_o[k]), k
+    assert out_g["napm"] == out_o["napm"]
+
+
+def _resident_kick_then_drift(s):
+    p = s.p
+    p.upload(s.xe, s.ve, s.pe, s.re)
+    k = p.kick_resident(a_mid=s.a, dt=s.dt)
+    p.buffer_v_resident()
+    x1, v1, p1, r1 = p.download(cap=_cap(s.geo))
+    d = p.drift_resident(dt_old=s.dt, dt=s.dt, vz_max=k["vz_max"])
+    x2, v2, p2, r2 = p.download(cap=_cap(s.geo))
+    st, xo, vo, po, ro, oo = orc.drift(oparams(s.geo), x1, v1, p1, r1, s.dt, s.dt, k["vz_max"])
+    _log(f"512^3 izipx {s.geo.izipx} drift: nplocal {d['nplocal']} (oracle {oo['nplocal']}), nlayer {d['nlayer']}")
+    assert st == 0 and d["nplocal"] == oo["nplocal"] == s.geo.ng ** 3 and d["nlayer"] == oo["nlayer"]
+    assert np.array_equal(r2, ro)
+    assert np.array_equal(x2, xo)
+    assert np.array_equal(v2, vo)
+    assert np.array_equal(p2, po)
+    assert np.array_equal(np.sort(p2), np.arange(1, s.geo.ng ** 3 + 1))
+
+
+def test_cfg2_512_drift_bit_exact(s512):
+    _resident_kick_then_drift(s512)
+
+
+def test_cfg2_512_int8_full_kick_and_drift():
+    s = State512(1)
+    try:
+        v_g = s.ve.copy()
+        out_g = s.p.kick(s.xe, v_g, s.re, a_mid=s.a, dt=s.dt)
+        v_o = s.ve.copy()
+        out_o, _, _ = orc.kick(oparams(s.geo), s.xe, v_o, s.re, s.tfs, s.a, s.dt)
+        err = rel_rms(v_g - s.ve, v_o - s.ve)
+        _log(f"512^3 int8 full kick: rms rel dv error {err:.3e}")
+        assert err <= 1e-5 and out_g["napm"] == out_o["napm"]
+        _resident_kick_then_drift(s)
+    finally:
+        s.p.close()
+
+
+def _global_positions(geo, xp, rhoc):
+    """Global coarse-cell coordinates of a physical layout (tile-major, cell
+    i fastest, particles in storage order), x = tile*nt + (i-1) + frac."""
+    counts = rhoc.reshape(-1).astype(np.int64)
+    cell = np.repeat(np.arange(counts.size, dtype=np.int64), counts)
+    frac = orc.decode(xp, geo.izipx)
+    out = np.empty((cell.size, 3), np.float64)
+    nte, nt, nnt, ncb = geo.nte, geo.nt, geo.nnt, geo.ncb
+    for d in range(3):
+        c = (cell // nte ** d) % nte
+        t = (cell // (nte ** 3 * nnt ** d)) % nnt
+        out[:, d] = t * nt + (c - ncb) + frac[:, d]
+    return out
+
+
+def _by_pid(geo, xp, vp, pid, rhoc):
+    x = _global_positions(geo, xp, rhoc)
+    inv = np.empty(pid.size, np.int64)
+    inv[pid - 1] = np.arange(pid.size)
+    return x[inv], vp[inv]
+
+
+def test_cfg1_128_ten_fixed_steps():
+    ng, box, z0, z1, nsteps = 128, 200.0, 49.0, 10.0, 10
+    geo = Geometry(ng, izipx=1)
+    P = oparams(geo, ires_fixed=3)
+    tfs = orc.tf_default(P)
+    p = P3M(ng, izipx=1, box=box, ires_fixed=3)
+    try:
+        ic = p.make_ic(seed=SEED + 1, z=z0, nic=2)
+        steps = cosmology.fixed_step_schedule(cosmology.ExpansionHistory.lcdm(z0), z1, nsteps)
+        cap = _cap(geo)
+        own = p.download(cap=cap)  # the oracle's independent run starts from the same input
+        vz_g = vz_o = ic["vz_max"]
+        dt_old = 0.0
+        for n, st in enumerate(steps):
+            dt, a_mid = st["dt"], st["a_mid"]
+            pre = p.download(cap=cap)
+            d, k = p.step_resident(dt_old=dt_old, dt=dt, vz_max=vz_g, a_mid=a_mid)
+            xg, vg, pg, rg = p.download(cap=cap)
+            # the oracle on the GPU's own pre-step state
+            s0, xo, vo, po, ro, oo = orc.drift(P, *pre, dt_old, dt, vz_g)
+            assert s0 == 0 and oo["nplocal"] == d["nplocal"] == ng ** 3 and oo["nlayer"] == d["nlayer"]
+            xb, vb, pb, rb = orc.buffer(P, xo, vo, po, ro, cap)
+            assert np.array_equal(rb, rg) and np.array_equal(xb, xg) and np.array_equal(pb, pg)
+            v_before = vb.copy()
+            ko, _, _ = orc.kick(P, xb, vb, rb, tfs, a_mid, dt)
+            orc.buffer_v(P, vb, rb)
+            err = rel_rms(vg - v_before, vb - v_before)
+            _log(f"cfg1 step {n + 1}: a_mid {a_mid:.5f} dt {dt:.4f} nlayer {d['nlayer']} napm {k['napm']} "
+                 f"rms rel dv error {err:.3e}")
+            assert k["napm"] == ko["napm"] and k["napm"][2] == geo.nnt ** 3 * geo.nns ** 3
+            assert err <= 1e-5
+            for key in ("dt_pm1", "dt_pm2", "dt_pm3", "dt_vmax", "vz_max"):
+                assert abs(k[key] - ko[key]) <= 1e-5 * abs(ko[key]), key
+            # dt_pp comes from the largest PP force.  Near r_pp the pair force
+            # F(r, app) - F(r, r_pp) vanishes while the reference's FP32
+            # polynomial for F(r, r_pp) (kick.f90:360-374: terms up to
+            # 840 ep^2 = 3360 summed in real(4)) carries an absolute error of
+            # up to ~6e-3 / (35 r_pp^2) per pair.  On this near-lattice input
+            # (z = 49..10, r_pp = 0.875 fine cells at ratio_sf = 4) the pairs in
+            # range sit near r_pp, so the oracle's largest PP force is rounding
+            # noise in the first steps and carries ~1e-3 of it later (measured:
+            # the GPU's FP32 largest PP force is within 4% of its FP64 form's at
+            # step 1, the oracle's 6.7x off).  dt_pp is compared to 2e-3 once the largest PP
+            # force exceeds 1000x the noise, and not before; the 512^3 test
+            # holds it to 1e-5.
+            noise = 6e-3 / (35.0 * (3.5 / 4) ** 2) * a_mid * dt / (6 * np.pi)
+            if np.sqrt(ko["f2max_pp"]) > 1e3 * noise:
+                assert abs(k["dt_pp"] - ko["dt_pp"]) <= 2e-3 * abs(ko["dt_pp"])
+            else:
+                _log(f"cfg1 step {n + 1}: largest PP velocity change {np.sqrt(ko['f2max_pp']):.2e} is below 1000x the "
+                     f"reference's FP32 rounding of F_ra ({noise:.1e}); dt_pp {k['dt_pp']:.4g} vs {ko['dt_pp']:.4g} "
+                     "not compared")
+            # the oracle's own run of the same step
+            s1, x1, v1, p1, r1, _ = orc.drift(P, *own, dt_old, dt, vz_o)
+            assert s1 == 0
+            x1, v1, p1, r1 = orc.buffer(P, x1, v1, p1, r1, cap)
+            k1, _, _ = orc.kick(P, x1, v1, r1, tfs, a_mid, dt)
+            orc.buffer_v(P, v1, r1)
+            own = (x1, v1, p1, r1)
+            vz_g, vz_o, dt_old = k["vz_max"], k1["vz_max"], dt
+        # closing half drift (dt_old = 0) puts positions at z_end
+        dg = p.drift_resident(dt_old=0.0, dt=dt_old, vz_max=vz_g)
+        xg, vg, pg, rg = p.download(cap=cap)
+        s2, xo, vo, po, ro, _ = orc.drift(P, *own, 0.0, dt_old, vz_o)
+        assert s2 == 0 and dg["nplocal"] == ng ** 3
+        Xg, Vg = _by_pid(geo, xg, vg, pg, rg)
+        Xo, Vo = _by_pid(geo, xo, vo, po, ro)
+        dx = Xg - Xo
+        dx -= geo.nc * np.round(dx / geo.nc)
+        pos_err = float(np.sqrt(np.mean(dx * dx))) / geo.nc
+        v_err = rel_rms(Vg, Vo)
+        same = float(np.mean(np.all(np.abs(dx) < 1e-12, axis=1)))
+        cells = float(np.mean(rg == ro))
+        _log(f"cfg1 after {nsteps} steps: positions rms {pos_err:.3e} of the box, velocities rms rel {v_err:.3e}, "
+             f"identical counts in {cells:.6f} of the cells, identical coordinates {same:.6f}")
+        # Two independent 10-step runs: each step agrees to <= 1e-5 (checked
+        # above on identical inputs), but a velocity difference at that level
+        # moves a few int8 coordinates (2^-8 coarse cell) by one unit, and
+        # such a particle's PP pairs then see a different separation, so the
+        # runs drift apart in velocity at the 1e-4 level while positions stay
+        # within 1e-5 of the box.
+        assert pos_err <= 1e-5 and v_err <= 1e-3
+        assert cells >= 0.9999 and same >= 0.999
+    finally:
+        p.close()

@@ -172,3 +172,72 @@ def test_gpu_multistep_gradient_matches_fp64_autograd():
     assert abs(float(L) - float(L_ref)) <= 1e-5 * float(L_ref)
     assert e <= 1e-4
     pm.close()
+
+
+@pytest.mark.gpu
+def test_gpu_lattice_displacements_match_absolute_positions():
+    """A context with lattice = ng takes displacements from the ng^3 sites:
+    force, VJP and CIC density of the displacements s equal those of the
+    absolute positions q + s (FP32 rounding of q + s aside)."""
+    n, ng, B, nw = 32, 16, 2, 16
+    pa = pmdiff.PMForce(n, B, SOFT)
+    pl = pmdiff.PMForce(n, B, SOFT, lattice=ng)
+    try:
+        gen = torch.Generator().manual_seed(41)
+        s64 = torch.randn(B * ng ** 3, 3, generator=gen, dtype=torch.float64) * 1.5
+        q = pl.sites().repeat(B, 1)
+        x = (q + s64).float().cuda()
+        s = s64.float().cuda()
+        g = torch.randn(B * ng ** 3, 3, generator=gen).cuda()
+        acc_a, Fa = pa.force(x)
+        acc_l, Fl = pl.force(s)
+        assert _rel(acc_l.double(), acc_a.double()) <= 1e-5
+        assert _rel(pl.force_vjp(s, None, g).double(), pa.force_vjp(x, Fa, g).double()) <= 1e-4
+        assert _rel(pl.cic(s, nw, nw / n).double(), pa.cic(x, nw, nw / n).double()) <= 1e-5
+    finally:
+        pa.close()
+        pl.close()
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_at_the_cfg5_size_matches_fp64_model():
+    """The cfg5 shape with one seed: 128^3 particles from white noise on a
+    256^3 mesh (box 256 Mpc/h), 10 KDK steps from a = 0.1 to 1, loss = MSE of
+    the final 128^3 CIC density against a second seed's; the gradient with
+    respect to the 128^3 noise modes through the HIP forward and hand-written
+    adjoint kernels (FP32, displacements from the particle lattice,
+    checkpointed positions with the force meshes recomputed in the backward)
+    against the FP64 model's autograd gradient (run with torch's own float64
+    GPU kernels), <= 1e-4 relative."""
+    B, ng, n, nw, nsteps, box, amp = 1, 128, 256, 128, 10, 256.0, 0.5
+    k1, d, k2 = pmdiff.kdk_coefficients(0.1, 1.0, nsteps)
+    s = 4 * np.pi * ng ** 3 / n ** 3
+    k1, k2 = [c / s for c in k1], [c / s for c in k2]
+    pm = pmdiff.PMForce(n, B, SOFT, lattice=ng)
+    try:
+        G = torch.from_numpy(pm.green().astype(np.float64)).cuda()
+        gen = torch.Generator().manual_seed(31)
+        noise = torch.randn(B, ng, ng, ng, generator=gen, dtype=torch.float64)
+        z5 = torch.randn(1, ng, ng, ng, generator=gen, dtype=torch.float64)
+        with torch.no_grad():
+            x, v = pmdiff.zeldovich(z5.cuda(), box, n, amp)
+            x, v = ref.kdk(x, v, G, 1, n, k1, d, k2)
+            target = ref.cic(x, 1, nw, nw / n)
+        zr = noise.cuda().requires_grad_(True)
+        x, v = pmdiff.zeldovich(zr, box, n, amp)
+        x, v = ref.kdk(x, v, G, B, n, k1, d, k2)
+        L_ref = ((ref.cic(x, B, nw, nw / n) - target) ** 2).mean()
+        (g_ref,) = torch.autograd.grad(L_ref, zr)
+        del x, v
+        zg = noise.float().cuda().requires_grad_(True)
+        x, v = pmdiff.zeldovich(zg, box, n, amp, displacement=True)
+        x, v = pmdiff.kdk(x, v, pm, k1, d, k2)
+        L = ((pmdiff.cic_density(x, pm, nw, nw / n).double() - target) ** 2).mean()
+        L.backward()
+        e = _rel(zg.grad.double(), g_ref)
+        el = abs(float(L.detach()) - float(L_ref)) / float(L_ref)
+        print(f"cfg5 size: loss {float(L.detach()):.6e} vs {float(L_ref):.6e} (rel {el:.1e}); gradient rel {e:.2e}")
+        assert el <= 1e-5
+        assert e <= 1e-4
+    finally:
+        pm.close()

@@ -31,7 +31,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
+def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False, nic=2):
     """per_tile: a one-byte workspace budget, so every tile and sub-tile mesh
     (and every PP sub-tile) is deposited, transformed and interpolated by its
     own launches -- the plain per-tile form."""
@@ -39,7 +39,7 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
 
     p = P3M(ng, izipx=2, precision=precision, box=box, workspace_bytes=1 if per_tile else 0)
     t0 = time.time()
-    ic = p.make_ic(seed=20260101, z=z, nic=2)
+    ic = p.make_ic(seed=20260101, z=z, nic=nic)
     log(f"[bench] fp{precision} ng={ng}: IC {ic['nplocal']} particles in {time.time() - t0:.1f}s, vz_max {ic['vz_max']:.3f}")
     a = 1.0 / (1.0 + z)
     # step criteria from a zero-length kick (timestep.f90:34 takes their minimum)
@@ -68,11 +68,25 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False):
     p.synchronize()
     t = time.perf_counter() - t0
     rep = p.timing_report() if timing else {}
-    tuned = {k: v["calls"] for k, v in p.timing_report().items() if k.startswith("pp_unroll_level")}
+    tuned = autotune_picks(p.timing_report())
     p.close()
     log(f"[bench] fp{precision}: {steps} steps in {t:.3f}s ({1e3 * t / steps:.1f} ms/step)")
     return {"seconds": t, "n": nloc, "report": rep, "dt": dt, "autotune": tuned, "per_step": per_step,
             "pairs": pairs}
+
+
+def autotune_picks(rep):
+    """{choice: {"pick": c, "ms": {candidate: trial ms}}} from the timing
+    report's autotune lines."""
+    out = {}
+    for k, v in rep.items():
+        if k.startswith("autotune:"):
+            _, key, cand = k.split(":")
+            e = out.setdefault(key, {"pick": None, "ms": {}})
+            e["ms"][cand] = v["ms"]
+            if v["calls"] == 1:
+                e["pick"] = int(cand)
+    return out
 
 
 def measured_traffic(kernel, ng):
@@ -388,6 +402,7 @@ def main():
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--z", type=float, default=2.0)
     ap.add_argument("--box", type=float, default=600.0)
+    ap.add_argument("--nic", type=int, default=2, help="potential grid refinement of the input maker (ic.f90 nic)")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
     ap.add_argument("--cpu-stride", type=int, default=8,
@@ -419,7 +434,7 @@ def main():
         print(json.dumps({"error": f"--gpus {args.gpus}: this benchmark runs the one-image step on one GPU only; "
                                    "a multi-GPU (nn>1) decomposition is not implemented"}))
         sys.exit(2)
-    r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box)
+    r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box, nic=args.nic)
     value = r32["n"] * args.steps / r32["seconds"]
     roof, phases = roofline(r32["report"], args.steps, args.ng, r32["pairs"])
     fp64 = fp64_plain = None

@@ -5,8 +5,13 @@
 // PM3 (sub-tile meshes grouped by resolution, batched) and PP, in the order
 // of kick.f90:32-311; the drift and the nn=1 buffer layout complete a step.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <string>
 
 #include "../../include/cube_p3m.h"
 #include "cic.h"
@@ -65,6 +70,33 @@ inline float dt_from_f2(float c, float f2, float a_mid) {
 // coefficients stay FP64 on the GPU.
 inline double kick_coef(const cube_kick_in& in) { return ((in.a_mid * in.dt) / 6.0) / kPi; }
 
+// Autotuner picks persisted across runs: when the environment names a file in
+// CUBE_AUTOTUNE_FILE, a context reads the picks recorded there (lines
+// "key pick") and appends every pick it makes, so later runs skip the trials.
+std::mutex g_tune_file_mu;
+
+void tune_file_read(std::map<std::string, int>& picks) {
+  const char* f = std::getenv("CUBE_AUTOTUNE_FILE");
+  if (!f) return;
+  std::lock_guard<std::mutex> lk(g_tune_file_mu);
+  if (FILE* fp = std::fopen(f, "r")) {
+    char k[256];
+    int v;
+    while (std::fscanf(fp, "%255s %d", k, &v) == 2) picks[k] = v;
+    std::fclose(fp);
+  }
+}
+
+void tune_file_append(const std::string& key, int pick) {
+  const char* f = std::getenv("CUBE_AUTOTUNE_FILE");
+  if (!f) return;
+  std::lock_guard<std::mutex> lk(g_tune_file_mu);
+  if (FILE* fp = std::fopen(f, "a")) {
+    std::fprintf(fp, "%s %d\n", key.c_str(), pick);
+    std::fclose(fp);
+  }
+}
+
 }  // namespace
 
 struct cube_p3m_ctx {
@@ -121,10 +153,21 @@ struct cube_p3m_ctx {
   DevBuf<int32_t> physcount;
   DevBuf<int64_t> physoff;
   int64_t last_pairs = 0;
-  // autotuned PP load depth per level (0 = not tuned yet) and its scratch
-  int pp_unroll[6] = {0, 0, 0, 0, 0, 0};
+  // autotuner: candidate times (ms) of every choice this context tuned, and
+  // the scratch the PP trials run on
+  struct TuneRecord {
+    std::vector<int> cand;
+    std::vector<float> ms;
+    int pick = 0;
+  };
+  std::map<std::string, TuneRecord> tuned;
+  std::map<std::string, int> tune_picks;  // picks of this context (and of CUBE_AUTOTUNE_FILE)
   DevBuf<float> vtune;
   DevBuf<PPOut> otune;
+  DevBuf<unsigned long long> tune_sum;
+  template <class Run>
+  int autotune(const std::string& key, const std::vector<int>& cand, Run&& run, const void* result,
+               size_t result_bytes);
 
   ~cube_p3m_ctx() {
     for (int i = 0; i < 7; ++i) {
@@ -186,6 +229,10 @@ struct cube_p3m_ctx {
   void pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h, const cube_kick_in& in,
                 PPOut* o);
   int tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S);
+  std::string tune_key(const char* what, int level, int64_t size) const {
+    return std::string(what) + "_l" + std::to_string(level) + "_n" + std::to_string(size) + "_fp" +
+           std::to_string(prec);
+  }
   void drift(const cube_drift_in& in, cube_drift_out& out);
   void buffer(int64_t* np_ext);
   void buffer_vel();
@@ -216,6 +263,7 @@ void cube_p3m_ctx::init() {
     else fcd[l].init(sizes[l]);
     tf[l].ensure((size_t)(sizes[l] / 2 + 1) * sizes[l] * sizes[l]);
   }
+  tune_file_read(tune_picks);
   pmout.ensure(4);
   ppout.ensure(1);
   napm.ensure(8);
@@ -283,7 +331,13 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
         const int64_t total = pm_dep_bin_count((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
         S.ents = depents.ensure((size_t)std::max<int64_t>(total, 1));
         pm_dep_bin_fill((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
-        pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s);
+        // threads per 16^3 block: the deposit writes every cell of its blocks
+        // from integer sums, so the trials leave the mesh as one run would
+        const int thr = autotune(
+            tune_key("dep_list_threads", level, n), {512, 256, 1024},
+            [&](int t) { pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s, t); }, mesh,
+            sizeof(R) * (size_t)(nreal * nb));
+        pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s, thr);
       } else
         pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, s);
     }
@@ -394,19 +448,20 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
   }
 }
 
-// Autotuner of the FP32 PP kernel's load depth (candidates in flight per
-// lane).  On the first batch of each PP level, every depth runs twice on a
-// scratch copy of the velocities and of the outputs, timed with HIP events on
-// the library stream; the smallest depth within 1% of the fastest is kept for
-// the rest of the context's life (a deeper one holds more registers, which
-// the kernel's 64-VGPR bound turns into spill traffic, for a gain inside the
-// timing noise).  All depths sum the pairs in the same order, so the choice
-// changes the time of the step only.
-int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
-  if (pp_unroll[iapm]) return pp_unroll[iapm];
-  static const int cand[3] = {4, 6, 8};
-  vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
-  otune.ensure(1);
+// Autotuner.  A choice whose candidates all give the same result (the same
+// sums in the same order, or integer sums) is timed on its first use: every
+// candidate runs twice, timed with HIP events on the library stream, and the
+// first candidate (in the caller's order of preference) within 1% of the
+// fastest is kept for the context (and in CUBE_AUTOTUNE_FILE when set).  The
+// candidate times are kept for the timing report.  After each trial the
+// result buffer is checksummed; candidates that disagree stop the run with an
+// error, so a pick can never change a result.
+template <class Run>
+int cube_p3m_ctx::autotune(const std::string& key, const std::vector<int>& cand, Run&& run, const void* result,
+                           size_t result_bytes) {
+  auto known = tune_picks.find(key);
+  if (known != tune_picks.end()) return known->second;
+  int pick = 0;
   struct Ev {  // destroyed on every exit, errors included
     hipEvent_t e = nullptr;
     ~Ev() {
@@ -415,27 +470,53 @@ int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
   } a, b;
   CUBE_HIP(hipEventCreate(&a.e));
   CUBE_HIP(hipEventCreate(&b.e));
-  hipEvent_t e0 = a.e, e1 = b.e;
-  float tbest[3] = {0, 0, 0};
+  std::vector<float> best(cand.size(), 0.0f);
+  uint64_t sum0 = 0;
   for (int rep = 0; rep < 2; ++rep)
-    for (int ci = 0; ci < 3; ++ci) {
-      const int u = cand[ci];
-      CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
-      zero_async(otune.p, sizeof(PPOut), s);
-      CUBE_HIP(hipEventRecord(e0, s));
-      pp_force(L, 32, ppboxes.p, nb, vtune.p, S, otune.p, s, u);
-      CUBE_HIP(hipEventRecord(e1, s));
-      CUBE_HIP(hipEventSynchronize(e1));
+    for (size_t ci = 0; ci < cand.size(); ++ci) {
+      CUBE_HIP(hipEventRecord(a.e, s));
+      run(cand[ci]);
+      CUBE_HIP(hipEventRecord(b.e, s));
+      CUBE_HIP(hipEventSynchronize(b.e));
       float ms = 0;
-      CUBE_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (tbest[ci] == 0 || ms < tbest[ci]) tbest[ci] = ms;
+      CUBE_HIP(hipEventElapsedTime(&ms, a.e, b.e));
+      if (best[ci] == 0 || ms < best[ci]) best[ci] = ms;
+      const uint64_t sum = device_checksum(result, result_bytes, tune_sum.ensure(1), s);
+      if (rep == 0 && ci == 0) sum0 = sum;
+      else if (sum != sum0)
+        throw Error(CUBE_ERR_HIP, "autotune " + key + ": candidate " + std::to_string(cand[ci]) +
+                                      " gives a different result");
     }
-  const float best = std::min({tbest[0], tbest[1], tbest[2]});
-  int pick = cand[2];
-  for (int ci = 2; ci >= 0; --ci)
-    if (tbest[ci] <= 1.01f * best) pick = cand[ci];
-  pp_unroll[iapm] = pick;
+  const float fastest = *std::min_element(best.begin(), best.end());
+  pick = cand.back();
+  for (size_t ci = 0; ci < cand.size(); ++ci)
+    if (best[ci] <= 1.01f * fastest) {
+      pick = cand[ci];
+      break;
+    }
+  tuned[key] = TuneRecord{cand, best, pick};
+  tune_picks[key] = pick;
+  tune_file_append(key, pick);
   return pick;
+}
+
+// PP load depth (candidates in flight per lane), tuned per PP level on the
+// level's first batch.  The trials run on a scratch copy of the velocities and
+// of the outputs.  The preference order 6, 4, 8 keeps the middle depth unless
+// another is more than 1% faster (a deeper one holds more registers, which the
+// kernel's 64-VGPR bound turns into spill traffic).  All depths sum the pairs
+// in the same order, so the choice changes the time of the step only.
+int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
+  vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
+  otune.ensure(1);
+  return autotune(
+      tune_key("pp_depth", iapm, L.npgrid), {6, 4, 8},
+      [&](int u) {
+        CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
+        zero_async(otune.p, sizeof(PPOut), s);
+        pp_force(L, 32, ppboxes.p, nb, vtune.p, S, otune.p, s, u);
+      },
+      vtune.p, sizeof(float) * 3 * (size_t)np);
 }
 
 template <typename R>
@@ -1147,11 +1228,20 @@ int cube_p3m_timing_report(cube_p3m_ctx* ctx, char* buf, int64_t len) {
     char extra[128];
     snprintf(extra, sizeof(extra), "pp_pairs_last %lld 0 0 0\n", (long long)ctx->last_pairs);
     r += extra;
-    for (int l = 2; l <= 5; ++l)
-      if (ctx->pp_unroll[l]) {
-        snprintf(extra, sizeof(extra), "pp_unroll_level%d %d 0 0 0\n", l, ctx->pp_unroll[l]);
+    for (const auto& kv : ctx->tune_picks)  // picks read from CUBE_AUTOTUNE_FILE, not timed here
+      if (!ctx->tuned.count(kv.first)) {
+        snprintf(extra, sizeof(extra), "autotune:%s:%d 1 0 0 0\n", kv.first.c_str(), kv.second);
         r += extra;
       }
+    for (const auto& kv : ctx->tuned) {
+      // "autotune:<key> <pick> <ms of the pick> 0 0", then one line per candidate
+      const auto& t = kv.second;
+      for (size_t ci = 0; ci < t.cand.size(); ++ci) {
+        snprintf(extra, sizeof(extra), "autotune:%s:%d %d %.4f 0 0\n", kv.first.c_str(), t.cand[ci],
+                 t.cand[ci] == t.pick ? 1 : 0, t.ms[ci]);
+        r += extra;
+      }
+    }
     if (buf && len > 0) {
       std::strncpy(buf, r.c_str(), (size_t)len - 1);
       buf[len - 1] = 0;

@@ -91,9 +91,13 @@ int64_t pm_dep_bin_count(int izipx, const PMLevel& L, const PMBox* boxes_dev, in
                          const int32_t* rhoc, const int64_t* cell_off, DepBins& S, hipStream_t s);
 void pm_dep_bin_fill(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp,
                      const int32_t* rhoc, const int64_t* cell_off, DepBins& S, hipStream_t s);
+// Order-independent checksum of `bytes` (a multiple of 4) at p; scratch8 is
+// 8 bytes of device memory.  Synchronises the stream.
+uint64_t device_checksum(const void* p, size_t bytes, void* scratch8, hipStream_t s);
+// threads per 16^3 block: 256, 512 or 1024 (the same mesh for each)
 template <typename R>
 void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const DepBins& S, R* mesh,
-                      float mass, hipStream_t s);
+                      float mass, hipStream_t s, int threads = 512);
 
 // Exclusive scan of int32 counts into int64 offsets (n+1 outputs).
 void scan_counts(const int32_t* in, int64_t* out, int64_t n, void* tmp, size_t tmp_bytes, hipStream_t s);

@@ -411,13 +411,12 @@ __global__ void k_dep_cap(PMLevel L, const PMBox* __restrict__ boxes, int nbox, 
 // same regions as k_deposit's mesh-aligned blocks; each thread deposits one
 // listed particle at a time (taps outside the block skipped), then the block
 // writes its mesh cells.
-// 512 threads (eight wavefronts) share a block's 32 KB accumulator: twice the
-// wavefronts per CU of 256-thread workgroups to cover the LDS atomics' latency
-// (measured 11.9 -> 10.9 ms for the 512^3 PM3 deposit).
-constexpr int kDepListThreads = 512;
-
-template <typename R, int IZ>
-__global__ void __launch_bounds__(kDepListThreads) k_deposit_lists(PMLevel L, const PMBox* __restrict__ boxes,
+// NT threads share a block's 32 KB accumulator: more wavefronts per CU cover
+// the LDS atomics' latency (512 threads measured 11.9 -> 10.9 ms against 256
+// for the 512^3 PM3 deposit); NT in {256, 512, 1024} is picked by the
+// engine's autotuner (integer sums: every NT gives the same mesh).
+template <typename R, int IZ, int NT>
+__global__ void __launch_bounds__(NT) k_deposit_lists(PMLevel L, const PMBox* __restrict__ boxes,
                                                        const int64_t* __restrict__ start,
                                                        const int32_t* __restrict__ used,
                                                        const unsigned long long* __restrict__ ents,
@@ -1235,22 +1234,32 @@ void pm_dep_bin_fill(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, 
   launch_dep_bin<1>(izipx, L, boxes, nbox, xp, rhoc, off, S, s);
 }
 
+template <typename R, int IZ>
+static void launch_deposit_lists(int threads, int64_t grid, const PMLevel& L, const PMBox* boxes, const DepBins& S,
+                                 R* mesh, float mass, hipStream_t s) {
+  if (threads == 256)
+    hipLaunchKernelGGL((k_deposit_lists<R, IZ, 256>), dim3((unsigned)grid), dim3(256), 0, s, L, boxes, S.start,
+                       S.count, S.ents, mesh, mass);
+  else if (threads == 1024)
+    hipLaunchKernelGGL((k_deposit_lists<R, IZ, 1024>), dim3((unsigned)grid), dim3(1024), 0, s, L, boxes, S.start,
+                       S.count, S.ents, mesh, mass);
+  else
+    hipLaunchKernelGGL((k_deposit_lists<R, IZ, 512>), dim3((unsigned)grid), dim3(512), 0, s, L, boxes, S.start,
+                       S.count, S.ents, mesh, mass);
+}
+
 template <typename R>
 void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const DepBins& S, R* mesh,
-                      float mass, hipStream_t s) {
+                      float mass, hipStream_t s, int threads) {
   const int64_t grid = (int64_t)nbox * pm_dep_blocks(L);
-  if (izipx == 2)
-    hipLaunchKernelGGL((k_deposit_lists<R, 2>), dim3((unsigned)grid), dim3(kDepListThreads), 0, s, L, boxes,
-                       S.start, S.count, S.ents, mesh, mass);
-  else
-    hipLaunchKernelGGL((k_deposit_lists<R, 1>), dim3((unsigned)grid), dim3(kDepListThreads), 0, s, L, boxes,
-                       S.start, S.count, S.ents, mesh, mass);
+  if (izipx == 2) launch_deposit_lists<R, 2>(threads, grid, L, boxes, S, mesh, mass, s);
+  else launch_deposit_lists<R, 1>(threads, grid, L, boxes, S, mesh, mass, s);
   CUBE_HIP(hipGetLastError());
 }
 template void pm_deposit_lists<float>(int, const PMLevel&, const PMBox*, int, const DepBins&, float*, float,
-                                      hipStream_t);
+                                      hipStream_t, int);
 template void pm_deposit_lists<double>(int, const PMLevel&, const PMBox*, int, const DepBins&, double*, float,
-                                       hipStream_t);
+                                       hipStream_t, int);
 
 void pm_level_scale(PMLevel& L) {
   int e = 0;
@@ -1310,6 +1319,29 @@ __global__ void k_zero(uint32_t* p, int64_t n) {
 }
 __global__ void k_zero_bytes(unsigned char* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Order-independent 64-bit checksum of a buffer of 32-bit words: the sum of
+// word * odd(index), so equal buffers give equal sums and a changed word
+// changes the sum.
+__global__ void k_checksum(const uint32_t* __restrict__ w, int64_t n, unsigned long long* __restrict__ out) {
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (unsigned long long)w[i] * (2ull * (unsigned long long)i + 1ull);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+uint64_t device_checksum(const void* p, size_t bytes, void* scratch8, hipStream_t s) {
+  zero_async(scratch8, 8, s);
+  const int64_t n = (int64_t)(bytes / 4);
+  hipLaunchKernelGGL(k_checksum, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192))),
+                     dim3(256), 0, s, (const uint32_t*)p, n, (unsigned long long*)scratch8);
+  CUBE_HIP(hipGetLastError());
+  uint64_t v = 0;
+  CUBE_HIP(hipMemcpyAsync(&v, scratch8, 8, hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  return v;
 }
 
 void zero_async(void* p, size_t bytes, hipStream_t s) {

@@ -355,6 +355,7 @@ bool fft_convolve_plane(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, 
   // n^2 (n/2+1) planes of FP64 beyond 96 do not fit in LDS
   if constexpr (std::is_same<R, float>::value) {
     if (c.n == 128) return run_plane<R, 128>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    if (c.n == 144) return run_plane<R, 144>(c, rho, work, G, nbox, div3, div1, s, tm, t);  // PM3 at 1024^3
     if (c.n == 160) return run_plane<R, 160>(c, rho, work, G, nbox, div3, div1, s, tm, t);
   }
   return false;

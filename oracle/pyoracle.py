@@ -154,16 +154,23 @@ def classify(P, rhoc):
     return ires, napm
 
 
-def tf_default(P):
-    """tf1, tf2, tf3_2..tf3_8 as initialize.f90:248-334 builds them."""
+def tf_default(P, levels=(1, 2, 3, 4, 5, 6)):
+    """tf1, tf2, tf3_2..tf3_8 as initialize.f90:248-334 builds them.  Levels
+    not in `levels` (1 = tf1, 2 = tf2, 3..6 = tf3_2..tf3_8) are returned as
+    zeros of the right shape, for kicks whose phase mask does not read them."""
     nc = P.ng // P.ratio_cs
     ngp = P.ng // P.nnt
     ngt = ngp + 2 * P.ngb
     sfs = (2, 4, 6, 8)
     nft = [(ngp // P.nns) * s + 2 * P.ratio_cs * s for s in sfs]
-    tfs = [green(nc, P.apm1c, 0.0, float(P.ratio_cs)), green(ngt, P.apm2, np.float32(P.apm1c) * np.float32(P.ratio_cs), 1.0)]
-    for s, n in zip(sfs, nft):
-        tfs.append(green(n, P.apm3f, np.float32(P.apm2) * np.float32(s), np.float32(1.0) / np.float32(s)))
+
+    def g(level, n, *a):
+        return green(n, *a) if level in levels else np.zeros((n, n, n // 2 + 1), np.float32)
+
+    tfs = [g(1, nc, P.apm1c, 0.0, float(P.ratio_cs)),
+           g(2, ngt, P.apm2, np.float32(P.apm1c) * np.float32(P.ratio_cs), 1.0)]
+    for lv, (s, n) in enumerate(zip(sfs, nft), start=3):
+        tfs.append(g(lv, n, P.apm3f, np.float32(P.apm2) * np.float32(s), np.float32(1.0) / np.float32(s)))
     if P.mass_nu > 0:
         tfs = [(np.float64(1 - P.f_nu) * t.astype(np.float64)).astype(np.float32) for t in tfs]
     return tfs

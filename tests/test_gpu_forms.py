@@ -83,17 +83,19 @@ def test_buffer_bit_exact(form):
     assert np.array_equal(xg[:m], xe) and np.array_equal(vg[:m], ve) and np.array_equal(pg[:m], pe)
 
 
-def test_pm2_with_80_point_tiles_matches_oracle():
-    """At ng = 192 the PM2 tile meshes are 80^3 (ngt = ng/4 + 2 ngb): the
-    size-specialised FFT passes at the PM3 size of the 512^3 step (two z
-    slices per workgroup) against the oracle, which the 64^3 cases (meshes of
-    24 to 96 points) do not reach at n = 80."""
-    geo = Geometry(192, izipx=2)
-    assert geo.ngt == 80
-    data = buffered_set(geo, seed=17, mean=6.0)
+@pytest.mark.parametrize("ng,n,mean", [(192, 80, 6.0), (448, 144, 1.0)])
+def test_pm2_with_80_and_144_point_tiles_matches_oracle(ng, n, mean):
+    """The PM2 tile meshes are ngt = ng/4 + 2 ngb points: 80^3 at ng = 192
+    (the PM3 size of the 512^3 step, two z slices per workgroup) and 144^3
+    at ng = 448 (the PM3 level-2 size of the 1024^3 step), both on the
+    size-specialised FFT passes, against the oracle; the 64^3 cases (meshes
+    of 24 to 96 points) reach neither size."""
+    geo = Geometry(ng, izipx=2)
+    assert geo.ngt == n
+    data = buffered_set(geo, seed=17, mean=mean)
     P = oparams(geo)
-    tfs = orc.tf_default(P)
-    p = P3M(192, izipx=2, precision=32)
+    tfs = orc.tf_default(P, levels=(2,))  # the PM2 phase reads tf2 only
+    p = P3M(ng, izipx=2, precision=32)
     try:
         xp, vp, pid, rhoc = data
         v_gpu = vp.copy()
@@ -101,7 +103,7 @@ def test_pm2_with_80_point_tiles_matches_oracle():
         out_g = p.kick(xp, v_gpu, rhoc, a_mid=A_MID, dt=DT, phase_mask=2)
         out_o, _, _ = orc.kick(P, xp, v_orc, rhoc, tfs, A_MID, DT, phase_mask=2)
         err = rel_rms(v_gpu - vp, v_orc - vp)
-        print(f"PM2 at 192^3: rms rel dv err {err:.3e}")
+        print(f"PM2 at {ng}^3 ({n}-point tiles): rms rel dv err {err:.3e}")
         assert err <= 1e-5
         assert abs(out_g["f2max_pm2"] - out_o["f2max_pm2"]) <= 1e-5 * abs(out_o["f2max_pm2"])
     finally:

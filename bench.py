@@ -31,13 +31,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False, nic=2):
+def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False, nic=2, graph=False, digest=False):
     """per_tile: a one-byte workspace budget, so every tile and sub-tile mesh
     (and every PP sub-tile) is deposited, transformed and interpolated by its
-    own launches -- the plain per-tile form."""
+    own launches -- the plain per-tile form.  graph: the kick's compute pass
+    replayed from a HIP graph.  digest: a hash of the resident state after
+    the timed steps (outside the timed region)."""
     from cube_nu_amd import P3M
 
     p = P3M(ng, izipx=2, precision=precision, box=box, workspace_bytes=1 if per_tile else 0)
+    if graph:
+        p.graph(True)
     t0 = time.time()
     ic = p.make_ic(seed=20260101, z=z, nic=nic)
     log(f"[bench] fp{precision} ng={ng}: IC {ic['nplocal']} particles in {time.time() - t0:.1f}s, vz_max {ic['vz_max']:.3f}")
@@ -68,11 +72,23 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False, 
     p.synchronize()
     t = time.perf_counter() - t0
     rep = p.timing_report() if timing else {}
-    tuned = autotune_picks(p.timing_report())
+    full = p.timing_report()
+    tuned = autotune_picks(full)
+    gl = {"captures": full["graph_captures"]["calls"], "launches": full["graph_launches"]["calls"]}
+    h = None
+    if digest:
+        import hashlib
+
+        import numpy as np
+
+        hh = hashlib.blake2b(digest_size=16)
+        for a in p.download():
+            hh.update(np.ascontiguousarray(a).tobytes())
+        h = hh.hexdigest()
     p.close()
-    log(f"[bench] fp{precision}: {steps} steps in {t:.3f}s ({1e3 * t / steps:.1f} ms/step)")
+    log(f"[bench] fp{precision}{' graph' if graph else ''}: {steps} steps in {t:.3f}s ({1e3 * t / steps:.1f} ms/step)")
     return {"seconds": t, "n": nloc, "report": rep, "dt": dt, "autotune": tuned, "per_step": per_step,
-            "pairs": pairs}
+            "pairs": pairs, "graph": gl, "digest": h}
 
 
 def autotune_picks(rep):
@@ -394,6 +410,85 @@ def precision_study(ng, z, box):
     return out
 
 
+def precision_study_nu(ng, z, box, m_nu_total=0.06, n_species=3):
+    """CDM + neutrino particles, FP32 form against the FP64 form per field.
+
+    ng^3 CDM and ng^3 neutrino particles from the same seeded Zel'dovich input
+    (the neutrinos start on the CDM positions), the neutrinos with seeded
+    Fermi-Dirac thermal velocities of n_species equal-mass species of
+    m_nu_total / n_species eV each, particle masses (1 - f_nu) and f_nu; the
+    neutrinos are PM-only (PM1 + PM2).  Fields: each kick phase's velocity
+    change (from zero velocities) for both species, the full kick, and after a
+    full kick + buffer_v + drift the positions and velocities matched by
+    particle ID."""
+    import numpy as np
+    from cube_nu_amd import P3M, cosmology
+
+    a = 1.0 / (1.0 + z)
+    fnu = cosmology.f_nu(m_nu_total)
+    m_c, m_n = 1.0 - fnu, fnu
+    vs = cosmology.nu_thermal_velocity_scale(m_nu_total / n_species, ng, box)
+    c = P3M(ng, izipx=2, precision=32, box=box)
+    n = P3M(ng, izipx=2, precision=32, box=box)
+    c.make_ic(seed=20260101, z=z, nic=2)
+    n.make_ic(seed=20260101, z=z, nic=2)
+    n.add_thermal_velocities(vs, seed=0x6e75_7468_6572_6d31)
+    k0, k0n = c.kick_resident_nu(n, a_mid=a, dt=0.0, nu_mass=m_n, mass_p_cdm=m_c)
+    dt = min(k0["dt_pm1"], k0["dt_pm2"], k0["dt_pm3"], k0["dt_pp"], k0["dt_vmax"], k0n["dt_vmax"], 1.0)
+    S = {"c": c.download(), "n": n.download()}
+    c.close()
+    n.close()
+    geo = c.geo
+    phases = (("pm1", 1), ("pm2", 2), ("pm3", 4), ("pp", 8), ("kick", 0))
+    res = {32: {}, 64: {}}
+    vz = None
+    for prec in (32, 64):
+        t0 = time.time()
+        q = {s: P3M(ng, izipx=2, precision=prec, box=box) for s in "cn"}
+        r = res[prec]
+        for name, mask in phases:
+            for s in "cn":
+                x, v, p, rc = S[s]
+                q[s].upload(x, np.zeros_like(v), p, rc)
+            q["c"].kick_resident_nu(q["n"], a_mid=a, dt=dt, nu_mass=m_n, mass_p_cdm=m_c, phase_mask=mask)
+            r["c_" + name] = q["c"].download()[1]
+            if mask in (0, 1, 2):
+                r["n_" + name] = q["n"].download()[1]
+        for s in "cn":
+            q[s].upload(*S[s])
+        kc, kn = q["c"].kick_resident_nu(q["n"], a_mid=a, dt=dt, nu_mass=m_n, mass_p_cdm=m_c)
+        vz = vz or {"c": kc["vz_max"], "n": kn["vz_max"]}  # the same colour counts in both drifts
+        for s in "cn":
+            q[s].buffer_v_resident()
+            q[s].drift_resident(dt_old=dt, dt=dt, vz_max=vz[s])
+            x2, v2, p2, rc2 = q[s].download(cap=S[s][0].shape[0])
+            q[s].close()
+            inv = np.empty(p2.size, np.int64)
+            inv[p2 - 1] = np.arange(p2.size)
+            r[s + "_x"] = _positions(geo, x2, rc2)[inv]
+            r[s + "_v"] = v2[inv]
+            r[s + "_xp"], r[s + "_rhoc"] = x2[inv], rc2
+        log(f"[bench] CDM + neutrino precision study fp{prec}: {time.time() - t0:.1f}s")
+    A, B = res[32], res[64]
+    out = {"config": f"{ng}^3 CDM + {ng}^3 neutrino particles, box {box:g} Mpc/h, z={z:g}, dt={dt:.4g}, "
+                     f"sum m_nu {m_nu_total:g} eV ({n_species} x {m_nu_total / n_species:g} eV), f_nu {fnu:.5f}, "
+                     f"thermal velocity scale {vs:.3f} (code units per u = pc/kT)",
+           "cdm": {}, "nu": {}}
+    for s, key in (("c", "cdm"), ("n", "nu")):
+        o = out[key]
+        for name, mask in phases:
+            if s + "_" + name in A:
+                o[f"dv_{name}"] = _rel(A[s + "_" + name], B[s + "_" + name])
+        o["v_after_step"] = _rel(A[s + "_v"], B[s + "_v"])
+        d = A[s + "_x"] - B[s + "_x"]
+        d -= geo.nc * np.round(d / geo.nc)
+        o["positions"] = {"identical_frac": float(np.mean(np.all(A[s + "_xp"] == B[s + "_xp"], axis=1))),
+                          "rhoc_identical": bool(np.array_equal(A[s + "_rhoc"], B[s + "_rhoc"])),
+                          "rms_dx_coarse_cells": float(np.sqrt(np.mean(d * d))),
+                          "max_dx_coarse_cells": float(np.abs(d).max())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -402,6 +497,7 @@ def main():
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--z", type=float, default=2.0)
     ap.add_argument("--box", type=float, default=600.0)
+    ap.add_argument("--graph", type=int, default=1, help="0 = skip the graph-on leg of the step")
     ap.add_argument("--nic", type=int, default=2, help="potential grid refinement of the input maker (ic.f90 nic)")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
@@ -411,8 +507,12 @@ def main():
     ap.add_argument("--precision-study", type=int, default=1,
                     help="0 = skip the per-field FP32-vs-FP64 comparison")
     ap.add_argument("--precision-z", type=float, default=None, help="redshift of that comparison (default --z)")
+    ap.add_argument("--nu-study", type=int, default=1,
+                    help="0 = skip the CDM + neutrino-particle FP32-vs-FP64 comparison (at z=0.5)")
     ap.add_argument("--precision-only", action="store_true",
                     help="only run the FP32-vs-FP64 comparison and print its JSON object")
+    ap.add_argument("--nu-only", action="store_true",
+                    help="only run the CDM + neutrino-particle comparison and print its JSON object")
     ap.add_argument("--pm-adjoint-only", action="store_true",
                     help="only run the PM forward+adjoint timing and print its JSON object")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -428,6 +528,9 @@ def main():
     if args.precision_only:
         print(json.dumps(precision_study(args.ng, zp, args.box)), flush=True)
         return
+    if args.nu_only:
+        print(json.dumps(precision_study_nu(args.ng, 0.5, args.box)), flush=True)
+        return
     if args.pm_adjoint:
         import torch  # noqa: F401  (imported first: P3M then lets torch's runtime initialise the device)
     if args.gpus != 1:
@@ -435,6 +538,22 @@ def main():
                                    "a multi-GPU (nn>1) decomposition is not implemented"}))
         sys.exit(2)
     r32 = run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box, nic=args.nic)
+    graph = None
+    if args.graph:
+        # the same steps from the same input, both without timing scopes: the
+        # kick's compute pass launched eagerly, then replayed from a HIP graph;
+        # the final resident states must be bit-identical
+        try:
+            legs = [run_form(args.ng, 32, args.steps, args.warmup, args.z, args.box, timing=False, nic=args.nic,
+                             graph=gr, digest=True) for gr in (False, True)]
+            graph = {"ms_per_step_off": 1e3 * legs[0]["seconds"] / args.steps,
+                     "median_ms_per_step_off": 1e3 * statistics.median(legs[0]["per_step"]),
+                     "ms_per_step_on": 1e3 * legs[1]["seconds"] / args.steps,
+                     "median_ms_per_step_on": 1e3 * statistics.median(legs[1]["per_step"]),
+                     "captures": legs[1]["graph"]["captures"], "launches": legs[1]["graph"]["launches"],
+                     "bit_identical_state": legs[0]["digest"] == legs[1]["digest"]}
+        except Exception as e:  # reported, not hidden
+            graph = {"error": str(e)}
     value = r32["n"] * args.steps / r32["seconds"]
     roof, phases = roofline(r32["report"], args.steps, args.ng, r32["pairs"])
     fp64 = fp64_plain = None
@@ -466,6 +585,12 @@ def main():
             prec = precision_study(args.ng, zp, args.box)
         except Exception as e:  # reported, not hidden
             prec = {"error": str(e)}
+    nu = None
+    if args.nu_study:
+        try:
+            nu = precision_study_nu(args.ng, 0.5, args.box)
+        except Exception as e:  # reported, not hidden
+            nu = {"error": str(e)}
     cpu = None
     if args.cpu_stride > 0:
         try:
@@ -483,7 +608,7 @@ def main():
                    "ng": args.ng, "izipx": 2, "dt": r32["dt"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64_form": fp64, "fp64_plain_per_tile": fp64_plain,
         "phase_ms_per_step": phases, "autotune": r32["autotune"], "pm_forward_adjoint": pm_adj,
-        "fp32_vs_fp64": prec,
+        "fp32_vs_fp64": prec, "cdm_nu_fp32_vs_fp64": nu, "graph": graph,
     }
     print(json.dumps(line), flush=True)
 

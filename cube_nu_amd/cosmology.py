@@ -242,3 +242,30 @@ def fixed_step_schedule(hist: ExpansionHistory, z_end, nsteps):
                       "dtau": float(dtau)})
         hist.advance(da, dtau)
     return steps
+
+
+# Neutrino particles (no reference counterpart: the reference treats massive
+# neutrinos by linear response, neutrinos/power_nu.f90).
+K_B_EV = 8.617333262e-5      # Boltzmann constant, eV/K
+T_NU0 = 1.9454               # neutrino temperature today, K ((4/11)^(1/3) T_CMB, T_CMB = 2.7255 K)
+C_KM_S = 299792.458
+NU_EV_PER_OMEGA_H2 = 93.14   # sum m_nu / (Omega_nu h^2), eV
+
+
+def f_nu(sum_m_nu_ev, h0=0.6817, omega_m=0.301):
+    """Neutrino fraction of the matter density, Omega_nu / Omega_m."""
+    return sum_m_nu_ev / NU_EV_PER_OMEGA_H2 / h0 ** 2 / omega_m
+
+
+def nu_thermal_velocity_scale(m_nu_ev, ng, box, omega_m=0.301):
+    """Code velocity of u = pc/(k T_nu) = 1 for neutrinos of mass m_nu (eV).
+
+    Non-relativistic, the peculiar velocity is u k T_nu / (m_nu c^2) c with
+    T_nu = T_nu0 (1+z).  The code velocity is dx/dtau in fine cells (box/ng
+    Mpc/h each) per unit of the time tau with dtau = dt / (a^2 t_u) and
+    t_u = 2 / (3 H0 sqrt(Omega_m)) (the time unit of H = 2/(3 a^1.5)
+    sqrt(1 + ...) in the Zel'dovich velocities' vfactor a^2 H, ic.f90:937-945,
+    and of omhsq0, parameters.f90:163), so v_pec = 150 sqrt(Omega_m)
+    (box/ng) v / a km/s; the factors a (1+z) = 1 cancel and the scale is the
+    same at every redshift."""
+    return K_B_EV * T_NU0 / m_nu_ev * C_KM_S * ng / (150.0 * math.sqrt(omega_m) * box)

@@ -70,6 +70,14 @@ inline float dt_from_f2(float c, float f2, float a_mid) {
 // coefficients stay FP64 on the GPU.
 inline double kick_coef(const cube_kick_in& in) { return ((in.a_mid * in.dt) / 6.0) / kPi; }
 
+inline uint64_t fbits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+__global__ void k_set_f64(double* p, double v) { *p = v; }
+
 // Autotuner picks persisted across runs: when the environment names a file in
 // CUBE_AUTOTUNE_FILE, a context reads the picks recorded there (lines
 // "key pick") and appends every pick it makes, so later runs skip the trials.
@@ -136,17 +144,20 @@ struct cube_p3m_ctx {
   DevBuf<char> scan_tmp;
   DevBuf<char> work;
   DevBuf<char> stencil;  // coarse-level deposit: 27 tap sums per coarse cell
-  DevBuf<PMBox> boxes;
-  DevBuf<PPBox> ppboxes;
+  // per-slot box lists and binned lists (slot 0 PM1, 1 PM2, 2..5 PM3 at
+  // iapm 2..5; PP by iapm 2..5): a captured kick reads each level's lists
+  // from where its sizing pass left them
+  DevBuf<PMBox> lboxes[6];
+  DevBuf<int32_t> ldepcount[6];
+  DevBuf<int64_t> ldepstart[6];
+  DevBuf<unsigned long long> ldepents[6];
+  DevBuf<PPBox> lppboxes[6];
+  DevBuf<int32_t> lppcount[6];
+  DevBuf<int64_t> lppstart[6];
+  DevBuf<float4> lppent[6];
   DevBuf<PMOut> pmout;
   DevBuf<PPOut> ppout;
   DevBuf<int32_t> ires, napm, cmax;
-  DevBuf<int32_t> depcount;
-  DevBuf<int64_t> depstart;
-  DevBuf<unsigned long long> depents;
-  DevBuf<int32_t> ppcount;
-  DevBuf<int64_t> ppstart;
-  DevBuf<float4> ppent;
   DevBuf<unsigned> ppclaim;
   DevBuf<int32_t> dflags, drank, dtable, dnewcount;
   DevBuf<int64_t> dbase;
@@ -169,7 +180,37 @@ struct cube_p3m_ctx {
   int autotune(const std::string& key, const std::vector<int>& cand, Run&& run, const void* result,
                size_t result_bytes);
 
+  // Kick graph.  With graph_on, a full resident kick runs in two passes: the
+  // sizing pass (classification, list counts and fills, box uploads, the
+  // autotuner) runs on the stream as usual and reads its sizes back; the
+  // compute pass (deposits, transforms, interpolation and kick, PP force,
+  // buffer_v) is captured into a graph once and replayed while the launch
+  // signature -- every level's box count and every buffer address the
+  // compute pass reads -- stays the same.  The kick coefficient, the one
+  // argument that changes every step, is read from device memory (kcoef).
+  bool graph_on = false;
+  int kph = 0;           // 0 both passes in one, 1 sizing only, 2 compute only
+  bool graph_ok = true;  // cleared by a sizing pass that needs more than one batch per level
+  std::vector<uint64_t> gsig_acc, gsig;
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_captures = 0, graph_launches = 0;
+  DevBuf<double> kcoef;
+  int64_t wsb = -1;  // workspace budget held for the duration of a kick
+  // A PM-only second species (neutrino particles) coupled for one kick: its
+  // particles deposit into the PM1 and PM2 meshes with mass nu_mass and take
+  // the PM1 and PM2 kicks; PM3 and PP act on this context's particles only.
+  cube_p3m_ctx* nu = nullptr;
+  float nu_mass = 0;
+  DevBuf<char> work2;  // the second species' level meshes before they are added
+  void sig_add(std::initializer_list<uint64_t> v) { gsig_acc.insert(gsig_acc.end(), v.begin(), v.end()); }
+  void graph_drop() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gsig.clear();
+  }
+
   ~cube_p3m_ctx() {
+    graph_drop();
     for (int i = 0; i < 7; ++i) {
       fcf[i].release();
       fcd[i].release();
@@ -214,6 +255,7 @@ struct cube_p3m_ctx {
   }
   int64_t ws_budget() const {
     if (cfg.workspace_bytes > 0) return cfg.workspace_bytes;
+    if (wsb >= 0) return wsb;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (int64_t)8 << 30;
     return std::max<int64_t>((int64_t)256 << 20, std::min<int64_t>((int64_t)24 << 30, (int64_t)(fr * 0.35)));
@@ -222,13 +264,16 @@ struct cube_p3m_ctx {
   void init();
   void set_tf_default();
   template <typename R>
-  void kick(const cube_kick_in& in, cube_kick_out& out, float* density_host);
+  void kick(const cube_kick_in& in, cube_kick_out& out, float* density_host, bool with_buffer_v = false);
   template <typename R>
-  void run_level(int level, const std::vector<PMBox>& bx, const float* G, const FFTCube<R>& fc, const PMLevel& L,
-                 const cube_kick_in& in, PMOut* o, const char* tag, float* density_host = nullptr);
+  void kick_pass(const cube_kick_in& in, cube_kick_out& out, float* density_host, int mask, bool with_buffer_v,
+                 std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h);
+  template <typename R>
+  void run_level(int level, int slot, const std::vector<PMBox>& bx, const float* G, const FFTCube<R>& fc,
+                 const PMLevel& L, const cube_kick_in& in, PMOut* o, const char* tag, float* density_host = nullptr);
   void pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h, const cube_kick_in& in,
                 PPOut* o);
-  int tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S);
+  int tune_pp(int iapm, const PPLevel& L, const PPBox* boxes, int nb, PPScratch& S);
   std::string tune_key(const char* what, int level, int64_t size) const {
     return std::string(what) + "_l" + std::to_string(level) + "_n" + std::to_string(size) + "_fp" +
            std::to_string(prec);
@@ -287,11 +332,12 @@ void cube_p3m_ctx::set_tf_default() {
 }
 
 template <typename R>
-void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const float* G, const FFTCube<R>& fc,
-                             const PMLevel& L0, const cube_kick_in& in, PMOut* o, const char* tag,
-                             float* density_host) {
+void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_in, const float* G,
+                             const FFTCube<R>& fc, const PMLevel& L0, const cube_kick_in& in, PMOut* o,
+                             const char* tag, float* density_host) {
   using C = typename Cplx<R>::T;
   if (bx_in.empty()) return;
+  const bool SZ = kph != 2, CP = kph != 1;  // sizing and compute parts of this call
   const int64_t n = L0.n;
   const int64_t nreal = n * n * n, ncplx = n * n * (n / 2 + 1);
   const int64_t per = nreal * (int64_t)sizeof(R) + ncplx * (int64_t)sizeof(C);
@@ -304,42 +350,73 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
   R* mesh = (R*)work.p;
   C* cplx = (C*)(work.p + batch * nreal * sizeof(R));
   const size_t bchunk = (level == 1) ? bx_in.size() : (size_t)batch;
+  if (kph == 1 && bchunk < bx_in.size()) graph_ok = false;  // a captured kick holds one batch per level
+  DevBuf<PMBox>& boxes = lboxes[slot];
   boxes.ensure(std::max<size_t>(boxes.n, bchunk));
   for (size_t b0 = 0; b0 < bx_in.size(); b0 += bchunk) {
     const int nb = (int)std::min<size_t>(bchunk, bx_in.size() - b0);
-    std::vector<PMBox> bx(bx_in.begin() + b0, bx_in.begin() + b0 + nb);
-    if (level != 1)
-      for (int i = 0; i < nb; ++i) bx[i].moff = (int64_t)i * nreal;
-    CUBE_HIP(hipMemcpyAsync(boxes.p, bx.data(), sizeof(PMBox) * nb, hipMemcpyHostToDevice, s));
     PMLevel L = L0;
     L.bstride = nreal;
     pm_level_scale(L);
+    if (kph == 2) L.kdev = kcoef.p;
     const int nmesh = (level == 1) ? 1 : nb;
     const double cells = (double)nreal * nmesh;
+    if (SZ) {
+      std::vector<PMBox> bx(bx_in.begin() + b0, bx_in.begin() + b0 + nb);
+      if (level != 1)
+        for (int i = 0; i < nb; ++i) bx[i].moff = (int64_t)i * nreal;
+      CUBE_HIP(hipMemcpyAsync(boxes.p, bx.data(), sizeof(PMBox) * nb, hipMemcpyHostToDevice, s));
+      sig_add({(uint64_t)level, (uint64_t)slot, (uint64_t)nb, (uint64_t)n, (uint64_t)(uintptr_t)boxes.p,
+               (uint64_t)(uintptr_t)work.p});
+    }
     {
       KScope k(&tm, (t + "deposit").c_str(), cells * sizeof(R));
-      if (level == 1)
-        pm1_deposit<R>((int)g.izipx, L, xpc(), rhocc(), off.p, stencil.ensure(pm1_stencil_bytes(L)), mesh,
-                       in.mass_p_cdm, o, s);
-      else if (level >= 3 && pm_deposit_binned_applies(L)) {
+      if (level == 1) {
+        void* st = stencil.ensure(pm1_stencil_bytes(L));
+        if (SZ) sig_add({(uint64_t)(uintptr_t)st});
+        if (CP)
+          pm1_deposit<R>((int)g.izipx, L, xpc(), rhocc(), off.p, st, mesh, in.mass_p_cdm, o, s);
+      } else if (level >= 3 && pm_deposit_binned_applies(L)) {
         const int64_t nblk = nb * pm_dep_blocks(L);
         DepBins S;
-        S.count = depcount.ensure((size_t)nblk);
-        S.start = depstart.ensure((size_t)nblk + 1);
+        S.count = ldepcount[slot].ensure((size_t)nblk);
+        S.start = ldepstart[slot].ensure((size_t)nblk + 1);
         S.scan_tmp = scan_scratch(nblk);
         S.scan_tmp_bytes = scan_tmp.n;
-        const int64_t total = pm_dep_bin_count((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
-        S.ents = depents.ensure((size_t)std::max<int64_t>(total, 1));
-        pm_dep_bin_fill((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
-        // threads per 16^3 block: the deposit writes every cell of its blocks
-        // from integer sums, so the trials leave the mesh as one run would
-        const int thr = autotune(
-            tune_key("dep_list_threads", level, n), {512, 256, 1024},
-            [&](int t) { pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s, t); }, mesh,
-            sizeof(R) * (size_t)(nreal * nb));
-        pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s, thr);
-      } else
+        if (SZ) {
+          const int64_t total = pm_dep_bin_count((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
+          ldepents[slot].ensure((size_t)std::max<int64_t>(total, 1));
+        }
+        S.ents = ldepents[slot].p;
+        if (SZ) pm_dep_bin_fill((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
+        // threads per 16^3 block: the FP32 deposit writes every cell of its
+        // blocks from integer sums, so the trials leave the mesh as one run
+        // would; the FP64 form sums FP64 values in LDS atomics, whose order
+        // (and last bits) follows the scheduling, and is not tuned
+        const int thr = std::is_same<R, float>::value
+                            ? autotune(tune_key("dep_list_threads", level, n), {512, 256, 1024},
+                                       [&](int t) {
+                                         pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm,
+                                                             s, t);
+                                       },
+                                       mesh, sizeof(R) * (size_t)(nreal * nb))
+                            : 512;
+        if (SZ) sig_add({(uint64_t)(uintptr_t)S.count, (uint64_t)(uintptr_t)S.start, (uint64_t)(uintptr_t)S.ents,
+                         (uint64_t)thr});
+        if (CP) pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm, s, thr);
+      } else if (CP)
         pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, s);
+    }
+    if (!CP) continue;
+    if (nu && level <= 2) {
+      KScope k(&tm, (t + "deposit_nu").c_str(), cells * sizeof(R));
+      R* mesh2 = (R*)work2.ensure(sizeof(R) * (size_t)(nreal * nmesh));
+      if (level == 1)
+        pm1_deposit<R>((int)g.izipx, L, nu->xpc(), nu->rhocc(), nu->off.p, stencil.p, mesh2, nu_mass,
+                       nu->pmout.p + 0, s);
+      else
+        pm_deposit<R>((int)g.izipx, L, boxes.p, nb, nu->xpc(), nu->rhocc(), nu->off.p, mesh2, nu_mass, s);
+      mesh_add<R>(mesh, mesh2, nreal * nmesh, s);
     }
     if (density_host) {
       std::vector<R> tmp(nreal);
@@ -354,11 +431,17 @@ void cube_p3m_ctx::run_level(int level, const std::vector<PMBox>& bx_in, const f
       KScope k(&tm, (t + "interp_kick").c_str(), cells * sizeof(R));
       pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, kick_coef(in), o, s);
     }
+    if (nu && level <= 2) {
+      KScope k(&tm, (t + "interp_kick_nu").c_str(), cells * sizeof(R));
+      pm_interp<R>((int)g.izipx, L, boxes.p, nb, nu->xpc(), nu->vpc(), nu->rhocc(), nu->off.p, mesh, kick_coef(in),
+                   nu->pmout.p + (level - 1), s);
+    }
   }
 }
 
 void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h,
                             const cube_kick_in& in, PPOut* o) {
+  const bool SZ = kph != 2, CP = kph != 1;
   // mass_scale and v_scale as kick.f90:569-574
   float mass_scale;
   if (cfg.pp_corr && cfg.mass_nu > 0)
@@ -373,7 +456,8 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
   // sub-tile's result does not depend on the order
   std::vector<int64_t> order(g.ns3);
   for (int64_t it = 0; it < g.ns3; ++it) order[it] = it;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cmax_h[a] > cmax_h[b]; });
+  if (SZ)
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cmax_h[a] > cmax_h[b]; });
   for (int iapm = 2; iapm <= 5; ++iapm) {
     std::vector<PPBox> bx;
     for (int64_t it : order)
@@ -404,6 +488,7 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
     L.app2 = L.app * L.app;
     L.mass_scale = mass_scale;
     L.v_scale = v_scale;
+    if (kph == 2) L.vdev = kcoef.p;
     L.inv_rcs = 1.0f / (float)L.rcs;
     {
       // r_pp in PP-cell units plus a margin of 1e-3 cell, far above the FP32
@@ -415,34 +500,44 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
     const int64_t m = L.npgrid + 2, m3 = m * m * m * kSX;
     const int64_t budget = ws_budget();
     int64_t batch = std::max<int64_t>(1, std::min<int64_t>((int64_t)bx.size(), budget / 4 / (m3 * 12)));
+    if (kph == 1 && batch < (int64_t)bx.size()) graph_ok = false;
+    DevBuf<PPBox>& boxes = lppboxes[iapm];
     for (size_t b0 = 0; b0 < bx.size(); b0 += (size_t)batch) {
       const int nb = (int)std::min<int64_t>(batch, (int64_t)(bx.size() - b0));
-      ppboxes.ensure(std::max<size_t>(ppboxes.n, (size_t)nb));
-      CUBE_HIP(hipMemcpyAsync(ppboxes.p, bx.data() + b0, sizeof(PPBox) * nb, hipMemcpyHostToDevice, s));
+      boxes.ensure(std::max<size_t>(boxes.n, (size_t)nb));
       PPScratch S;
-      S.count = ppcount.ensure((size_t)(nb * m3));
-      S.start = ppstart.ensure((size_t)(nb * m3 + 1));
+      S.count = lppcount[iapm].ensure((size_t)(nb * m3));
+      S.start = lppstart[iapm].ensure((size_t)(nb * m3 + 1));
       S.scan_tmp = scan_scratch(nb * m3);
       S.scan_tmp_bytes = scan_tmp.n;
-      S.claim = ppclaim.ensure(8);
-      int64_t total;
-      {
-        KScope k(&tm, "pp_bin_count", 0);
-        total = pp_bin_count((int)g.izipx, L, ppboxes.p, nb, xpc(), rhocc(), off.p, S, s);
+      S.claim = ppclaim.ensure(8);  // the force kernel's work-queue counters
+      int64_t total = 0;
+      if (SZ) {
+        CUBE_HIP(hipMemcpyAsync(boxes.p, bx.data() + b0, sizeof(PPBox) * nb, hipMemcpyHostToDevice, s));
+        {
+          KScope k(&tm, "pp_bin_count", 0);
+          total = pp_bin_count((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
+        }
+        // the force kernel queues 31-bit entry indices
+        if (total > (int64_t)INT32_MAX - kPPPad)
+          throw Error(CUBE_ERR_ARG,
+                      "PP batch of " + std::to_string(total) + " entries exceeds 2^31; lower workspace_bytes");
+        S.entries = lppent[iapm].ensure((size_t)(total + kPPPad));
+        S.capacity = total;
+        {
+          KScope k(&tm, "pp_bin_fill", (double)total * 16);
+          pp_bin_fill((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, S, s);
+        }
       }
-      // the force kernel queues 31-bit entry indices
-      if (total > (int64_t)INT32_MAX - kPPPad)
-        throw Error(CUBE_ERR_ARG, "PP batch of " + std::to_string(total) + " entries exceeds 2^31; lower workspace_bytes");
-      S.entries = ppent.ensure((size_t)(total + kPPPad));
-      S.capacity = total;
-      {
-        KScope k(&tm, "pp_bin_fill", (double)total * 16);
-        pp_bin_fill((int)g.izipx, L, ppboxes.p, nb, xpc(), rhocc(), off.p, S, s);
-      }
-      const int unroll = prec == 32 ? tune_pp(iapm, L, nb, S) : 6;
-      {
+      S.entries = lppent[iapm].p;
+      const int unroll = prec == 32 ? tune_pp(iapm, L, boxes.p, nb, S) : 6;
+      if (SZ)
+        sig_add({(uint64_t)iapm, (uint64_t)nb, (uint64_t)(uintptr_t)boxes.p, (uint64_t)(uintptr_t)S.start,
+                 (uint64_t)(uintptr_t)S.entries, (uint64_t)(uintptr_t)S.claim, (uint64_t)unroll,
+                 fbits(mass_scale)});
+      if (CP) {
         KScope k(&tm, "pp_force", (double)total * 16);
-        pp_force(L, prec, ppboxes.p, nb, vpc(), S, o, s, unroll);
+        pp_force(L, prec, boxes.p, nb, vpc(), S, o, s, unroll);
       }
     }
   }
@@ -506,7 +601,7 @@ int cube_p3m_ctx::autotune(const std::string& key, const std::vector<int>& cand,
 // another is more than 1% faster (a deeper one holds more registers, which the
 // kernel's 64-VGPR bound turns into spill traffic).  All depths sum the pairs
 // in the same order, so the choice changes the time of the step only.
-int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
+int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, const PPBox* boxes, int nb, PPScratch& S) {
   vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
   otune.ensure(1);
   return autotune(
@@ -514,19 +609,20 @@ int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, int nb, PPScratch& S) {
       [&](int u) {
         CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
         zero_async(otune.p, sizeof(PPOut), s);
-        pp_force(L, 32, ppboxes.p, nb, vtune.p, S, otune.p, s, u);
+        pp_force(L, 32, boxes, nb, vtune.p, S, otune.p, s, u);
       },
       vtune.p, sizeof(float) * 3 * (size_t)np);
 }
 
 template <typename R>
-void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* density_host) {
-  if (layout != 1) throw Error(CUBE_ERR_STATE, "kick needs the buffered (extended) particle layout");
-  std::memset(&out, 0, sizeof(out));
-  ensure_offsets();
-  const int mask = in.phase_mask ? in.phase_mask : 15;
-  zero_async(pmout.p, sizeof(PMOut) * 4, s);
-  zero_async(ppout.p, sizeof(PPOut), s);
+void cube_p3m_ctx::kick_pass(const cube_kick_in& in, cube_kick_out& out, float* density_host, int mask,
+                             bool with_buffer_v, std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h) {
+  const bool SZ = kph != 2, CP = kph != 1;
+  if (CP) {
+    zero_async(pmout.p, sizeof(PMOut) * 4, s);
+    zero_async(ppout.p, sizeof(PPOut), s);
+    if (nu) zero_async(nu->pmout.p, sizeof(PMOut) * 4, s);
+  }
   auto fc = [&](int l) -> const FFTCube<R>& {
     if constexpr (std::is_same<R, float>::value) return fcf[l];
     else return fcd[l];
@@ -549,7 +645,7 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
       b.moff = 0;
       bx.push_back(b);
     }
-    run_level<R>(1, bx, tf[1].p, fc(1), L, in, pmout.p + 0, "pm1_", density_host);
+    run_level<R>(1, 0, bx, tf[1].p, fc(1), L, in, pmout.p + 0, "pm1_", density_host);
   }
   // PM2 (kick.f90:47-69)
   if (mask & 2) {
@@ -564,11 +660,11 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
       for (int d = 0; d < 3; ++d) { b.nc1[d] = 1; b.nc2[d] = (int)g.nt; b.ushift[d] = 0; b.org[d] = 0; }
       bx.push_back(b);
     }
-    run_level<R>(2, bx, tf[2].p, fc(2), L, in, pmout.p + 1, "pm2_");
+    run_level<R>(2, 1, bx, tf[2].p, fc(2), L, in, pmout.p + 1, "pm2_");
   }
-  // classification (kick.f90:108-138)
-  std::vector<int32_t> ires_h(g.ns3, 2), cmax_h(g.ns3, 0);
-  if (mask & 12) {
+  // classification (kick.f90:108-138); a compute-only pass reuses the sizing
+  // pass's classes
+  if ((mask & 12) && SZ) {
     classify_subtiles(g, cfg.ires_fixed, rhocc(), ires.p, napm.p, cmax.p, s);
     CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipMemcpyAsync(cmax_h.data(), cmax.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
@@ -600,11 +696,101 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
       }
       char tag[32];
       snprintf(tag, sizeof(tag), "pm3_%d_", iapm);
-      run_level<R>(3, bx, tf[iapm + 1].p, fc(iapm + 1), L, in, pmout.p + 2, tag);
+      run_level<R>(3, iapm, bx, tf[iapm + 1].p, fc(iapm + 1), L, in, pmout.p + 2, tag);
     }
   }
   // PP (kick.f90:280-298)
   if ((mask & 8) && cfg.pp) pp_phase(ires_h, cmax_h, in, ppout.p);
+  // buffer_v after the kick (main.f90:38)
+  if (with_buffer_v && CP) {
+    KScope k(&tm, "buffer_v", (double)np * 12);
+    buffer_v(g, off.p, vpc(), nullptr, s);
+  }
+}
+
+template <typename R>
+void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* density_host, bool with_buffer_v) {
+  if (layout != 1) throw Error(CUBE_ERR_STATE, "kick needs the buffered (extended) particle layout");
+  std::memset(&out, 0, sizeof(out));
+  ensure_offsets();
+  const int mask = in.phase_mask ? in.phase_mask : 15;
+  std::vector<int32_t> ires_h(g.ns3, 2), cmax_h(g.ns3, 0);
+  struct Hold {  // the workspace budget is read once per kick, so both passes batch alike
+    int64_t& w;
+    ~Hold() { w = -1; }
+  } hold{wsb};
+  wsb = -1;
+  wsb = ws_budget();
+  const bool use_graph = graph_on && mask == 15 && !density_host && !nu;
+  if (!use_graph) {
+    kph = 0;
+    kick_pass<R>(in, out, density_host, mask, with_buffer_v, ires_h, cmax_h);
+  } else {
+    struct Phase {  // back to one-pass kicks on every exit, errors included
+      int& k;
+      ~Phase() { k = 0; }
+    } ph{kph};
+    const bool timing = tm.enabled;
+    {
+      KScope k(&tm, "kick_sizing", 0);
+      tm.enabled = false;
+      kph = 1;
+      graph_ok = true;
+      gsig_acc.clear();
+      try {
+        kick_pass<R>(in, out, nullptr, mask, with_buffer_v, ires_h, cmax_h);
+      } catch (...) {
+        tm.enabled = timing;
+        throw;
+      }
+      tm.enabled = timing;
+    }
+    kcoef.ensure(1);
+    hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, s, kcoef.p, kick_coef(in));
+    CUBE_HIP(hipGetLastError());
+    if (!graph_ok) {
+      // a level needs several workspace batches, whose lists share one set of
+      // buffers: the kick runs again in one pass, uncaptured (the sizing pass
+      // wrote lists and box tables only)
+      graph_drop();
+      kph = 0;
+      kick_pass<R>(in, out, nullptr, mask, with_buffer_v, ires_h, cmax_h);
+    } else {
+      kph = 2;
+      sig_add({(uint64_t)mask, (uint64_t)prec, fbits(in.mass_p_cdm), (uint64_t)with_buffer_v,
+               (uint64_t)(uintptr_t)xpc(), (uint64_t)(uintptr_t)vpc(), (uint64_t)(uintptr_t)rhocc(),
+               (uint64_t)(uintptr_t)off.p, (uint64_t)(uintptr_t)pmout.p, (uint64_t)(uintptr_t)ppout.p,
+               (uint64_t)(uintptr_t)kcoef.p});
+      KScope k(&tm, "kick_graph", 0);
+      if (!gexec || gsig_acc != gsig) {
+        graph_drop();
+        tm.enabled = false;
+        hipGraph_t gr = nullptr;
+        CUBE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        try {
+          kick_pass<R>(in, out, nullptr, mask, with_buffer_v, ires_h, cmax_h);
+        } catch (...) {
+          tm.enabled = timing;
+          hipGraph_t tmp = nullptr;
+          (void)hipStreamEndCapture(s, &tmp);
+          if (tmp) (void)hipGraphDestroy(tmp);
+          throw;
+        }
+        tm.enabled = timing;
+        CUBE_HIP(hipStreamEndCapture(s, &gr));
+        const hipError_t e = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+          gexec = nullptr;
+          throw Error(CUBE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        gsig = gsig_acc;
+        ++graph_captures;
+      }
+      CUBE_HIP(hipGraphLaunch(gexec, s));
+      ++graph_launches;
+    }
+  }
   // collect reductions
   PMOut po[4];
   PPOut pq;
@@ -911,6 +1097,47 @@ static void upload_ext(cube_p3m_ctx* c, const void* xp, const float* vp, const i
   c->vmax_valid = false;
 }
 
+// The second species' outputs of a coupled kick: its PM1/PM2 maxima (its
+// levels' f2max at its own positions, the Delta-t criteria from them, its
+// velocity maxima) and its PM1 density sum.
+template <typename R>
+static void nu_outputs(cube_p3m_ctx* ctx, cube_p3m_ctx* nu, const cube_kick_in& in, int mask, cube_kick_out& o) {
+  std::memset(&o, 0, sizeof(o));
+  PMOut pn[4];
+  CUBE_HIP(hipMemcpyAsync(pn, nu->pmout.p, sizeof(PMOut) * 4, hipMemcpyDeviceToHost, ctx->s));
+  CUBE_HIP(hipStreamSynchronize(ctx->s));
+  const bool f = std::is_same<R, float>::value;
+  auto f2 = [&](const PMOut& p) { return f ? p.f2max : (float)p.f2max_d; };
+  float vmax[3] = {0, 0, 0};
+  for (int lv = 0; lv < 2; ++lv) {
+    if (!(mask & (1 << lv))) continue;
+    const PMOut& p = pn[lv];
+    const float q = f2(p);
+    if (lv == 0) {
+      o.f2max_pm1 = q;
+      o.dt_pm1 = dt_from_f2(0.5f, q, (float)in.a_mid);
+      unsigned long long tfx = 0;
+      double td = 0;
+      for (int k = 0; k < kSumSlots; ++k) {
+        tfx += p.testrho_fx[k];
+        td += p.testrho_d[k];
+      }
+      o.testrho = f ? (double)tfx / 4294967296.0 : td;
+    } else {
+      o.f2max_pm2 = q;
+      o.dt_pm2 = dt_from_f2(0.5f, q, (float)in.a_mid);
+    }
+    for (int d = 0; d < 3; ++d) vmax[d] = f ? p.vmax[d] : (float)p.vmax_d[d];
+  }
+  for (int d = 0; d < 3; ++d) o.vmax[d] = vmax[d];
+  o.dt_vmax = (0.9f * 14.0f) / std::max({vmax[0], vmax[1], vmax[2]});
+  o.vz_max = vmax[2];
+  // PM2 interpolates and kicks every physical particle of the species, and
+  // buffer_v copies their velocities: the maxima bound its resident set
+  nu->vmax_valid = (mask & 2) != 0;
+  for (int d = 0; d < 3; ++d) nu->last_vmax[d] = vmax[d];
+}
+
 extern "C" {
 
 const char* cube_p3m_version(void) { return "cube_p3m 0.1 (gfx950)"; }
@@ -1140,6 +1367,52 @@ int cube_p3m_kick_resident(cube_p3m_ctx* ctx, const cube_kick_in* in, cube_kick_
   });
 }
 
+int cube_p3m_kick_resident_nu(cube_p3m_ctx* ctx, cube_p3m_ctx* nu, const cube_kick_in* in, float nu_mass,
+                              cube_kick_out* out, cube_kick_out* out_nu) {
+  return guarded(ctx, [&] {
+    if (!in || !nu) throw Error(CUBE_ERR_ARG, "null argument");
+    if (nu == ctx) throw Error(CUBE_ERR_ARG, "the second species needs a context of its own");
+    const cube_p3m_config &a = ctx->cfg, &b = nu->cfg;
+    if (nu->device != ctx->device || a.ng != b.ng || a.nnt != b.nnt || a.nns != b.nns || a.ratio_cs != b.ratio_cs ||
+        a.ncb != b.ncb || a.ngb != b.ngb || a.izipx != b.izipx || a.precision != b.precision)
+      throw Error(CUBE_ERR_ARG, "the two species' contexts differ in device, geometry, coordinates or precision");
+    if (nu->layout != 1) throw Error(CUBE_ERR_STATE, "the second species needs the buffered (extended) layout");
+    if (!(nu_mass >= 0.0f)) throw Error(CUBE_ERR_ARG, "nu_mass must be >= 0");
+    // the second species' arrays were last written on its own stream
+    CUBE_HIP(hipSetDevice(nu->device));
+    nu->ensure_offsets();
+    CUBE_HIP(hipStreamSynchronize(nu->s));
+    struct Detach {
+      cube_p3m_ctx* c;
+      ~Detach() { c->nu = nullptr; }
+    } detach{ctx};
+    ctx->nu = nu;
+    ctx->nu_mass = nu_mass;
+    const int mask = in->phase_mask ? in->phase_mask : 15;
+    cube_kick_out o, on;
+    if (ctx->prec == 32) {
+      ctx->kick<float>(*in, o, nullptr);
+      nu_outputs<float>(ctx, nu, *in, mask, on);
+    } else {
+      ctx->kick<double>(*in, o, nullptr);
+      nu_outputs<double>(ctx, nu, *in, mask, on);
+    }
+    if (out) *out = o;
+    if (out_nu) *out_nu = on;
+  });
+}
+
+int cube_p3m_add_thermal_velocities(cube_p3m_ctx* ctx, double v_scale, uint64_t seed, float* vz_max) {
+  return guarded(ctx, [&] {
+    if (ctx->layout == 0) throw Error(CUBE_ERR_STATE, "no resident particles");
+    if (!ctx->has_pid) throw Error(CUBE_ERR_STATE, "thermal velocities are keyed by particle ID: no resident IDs");
+    if (!(v_scale >= 0.0)) throw Error(CUBE_ERR_ARG, "v_scale must be >= 0");
+    const float vz = add_thermal_velocities(ctx->vpc(), ctx->pidc(), ctx->np, v_scale, seed, ctx->s);
+    ctx->vmax_valid = false;
+    if (vz_max) *vz_max = vz;
+  });
+}
+
 int cube_p3m_drift_resident(cube_p3m_ctx* ctx, const cube_drift_in* in, cube_drift_out* out) {
   return guarded(ctx, [&] {
     if (!in) throw Error(CUBE_ERR_ARG, "null argument");
@@ -1162,9 +1435,8 @@ int cube_p3m_step_resident(cube_p3m_ctx* ctx, const cube_drift_in* din, const cu
     int64_t n = 0;
     ctx->buffer(&n);
     cube_kick_out k;
-    if (ctx->prec == 32) ctx->kick<float>(*kin, k, nullptr);
-    else ctx->kick<double>(*kin, k, nullptr);
-    ctx->buffer_vel();
+    if (ctx->prec == 32) ctx->kick<float>(*kin, k, nullptr, true);  // buffer_v rides on the kick
+    else ctx->kick<double>(*kin, k, nullptr, true);
     if (dout) *dout = d;
     if (kout) *kout = k;
   });
@@ -1214,6 +1486,13 @@ int cube_p3m_synchronize(cube_p3m_ctx* ctx) {
   return guarded(ctx, [&] { CUBE_HIP(hipStreamSynchronize(ctx->s)); });
 }
 
+int cube_p3m_graph_enable(cube_p3m_ctx* ctx, int32_t enable) {
+  return guarded(ctx, [&] {
+    ctx->graph_on = enable != 0;
+    if (!ctx->graph_on) ctx->graph_drop();
+  });
+}
+
 int cube_p3m_timing_enable(cube_p3m_ctx* ctx, int32_t enable) {
   return guarded(ctx, [&] { ctx->tm.enabled = enable != 0; });
 }
@@ -1227,6 +1506,9 @@ int cube_p3m_timing_report(cube_p3m_ctx* ctx, char* buf, int64_t len) {
     std::string r = ctx->tm.report();
     char extra[128];
     snprintf(extra, sizeof(extra), "pp_pairs_last %lld 0 0 0\n", (long long)ctx->last_pairs);
+    r += extra;
+    snprintf(extra, sizeof(extra), "graph_captures %lld 0 0 0\ngraph_launches %lld 0 0 0\n",
+             (long long)ctx->graph_captures, (long long)ctx->graph_launches);
     r += extra;
     for (const auto& kv : ctx->tune_picks)  // picks read from CUBE_AUTOTUNE_FILE, not timed here
       if (!ctx->tuned.count(kv.first)) {

@@ -11,6 +11,7 @@
 // PID (ic.f90:725-727).  Unlike ic.f90, particles are binned globally, so
 // large displacements are kept.  P(k) = A k^ns T_BBKS(k)^2 normalised to
 // sigma8 with a top-hat of 8 Mpc/h, scaled by Dgrow(a)^2 (ic.f90:195-208).
+#include <cmath>
 #include <vector>
 
 #include "fft.h"
@@ -286,6 +287,76 @@ ICResult make_ic(const Geo& g, const cube_ic_params& p, void* xp, float* vp, int
   fc.release();
   res.nplocal = N;
   return res;
+}
+
+namespace {
+
+constexpr int kFDTable = 8192;  // inverse-CDF points, uniform in probability
+constexpr double kFDUmax = 40.0;
+
+// u at CDF = i/kFDTable for u^2/(e^u + 1) on [0, kFDUmax] (the mass beyond
+// 40 is 1e-15 of the total): Simpson integration on a fine grid in FP64,
+// then linear inversion.
+std::vector<double> fd_inverse_cdf() {
+  const int nf = 400000;
+  const double h = kFDUmax / nf;
+  std::vector<double> cdf(nf + 1, 0.0);
+  auto f = [](double u) { return u * u / (std::exp(u) + 1.0); };
+  for (int i = 0; i < nf; ++i) {
+    const double a = i * h;
+    cdf[i + 1] = cdf[i] + h / 6.0 * (f(a) + 4.0 * f(a + 0.5 * h) + f(a + h));
+  }
+  const double tot = cdf[nf];
+  std::vector<double> inv(kFDTable + 1);
+  int j = 0;
+  for (int i = 0; i <= kFDTable; ++i) {
+    const double c = tot * (double)i / kFDTable;
+    while (j < nf - 1 && cdf[j + 1] < c) ++j;
+    const double d = cdf[j + 1] - cdf[j];
+    inv[i] = (j + (d > 0 ? (c - cdf[j]) / d : 0.0)) * h;
+  }
+  inv[0] = 0.0;
+  return inv;
+}
+
+__global__ void k_thermal(float* __restrict__ vp, const int64_t* __restrict__ pid, int64_t n, uint64_t seed,
+                          double vscale, const double* __restrict__ icdf, float* __restrict__ vzmax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float vz = 0.0f;
+  if (i < n) {
+    const uint64_t key = (uint64_t)pid[i];
+    const double r1 = uniform01(seed, 3 * key), r2 = uniform01(seed, 3 * key + 1), r3 = uniform01(seed, 3 * key + 2);
+    const double t = r1 * kFDTable;
+    const int k = min((int)t, kFDTable - 1);
+    const double u = icdf[k] + (t - k) * (icdf[k + 1] - icdf[k]);
+    const double ct = 2.0 * r2 - 1.0, st = sqrt(fmax(0.0, 1.0 - ct * ct)), ph = 2.0 * kPi * r3;
+    const double w = u * vscale;
+    const double dv[3] = {w * st * cos(ph), w * st * sin(ph), w * ct};
+    for (int d = 0; d < 3; ++d) vp[3 * i + d] = (float)((double)vp[3 * i + d] + dv[d]);
+    vz = fabsf(vp[3 * i + 2]);
+  }
+  for (int o = 32; o > 0; o >>= 1) vz = fmaxf(vz, __shfl_down(vz, o, 64));
+  if ((threadIdx.x & 63) == 0 && vz > 0) atomic_max_nonneg(vzmax, vz);
+}
+
+}  // namespace
+
+float add_thermal_velocities(float* vp, const int64_t* pid, int64_t n, double v_scale, uint64_t seed,
+                             hipStream_t s) {
+  const std::vector<double> inv = fd_inverse_cdf();
+  Tmp<double> tab(kFDTable + 1);
+  Tmp<float> vz(1);
+  CUBE_HIP(hipMemcpyAsync(tab.p, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, s));
+  zero_async(vz.p, sizeof(float), s);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_thermal, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, vp, pid, n, seed, v_scale,
+                       tab.p, vz.p);
+    CUBE_HIP(hipGetLastError());
+  }
+  float out = 0;
+  CUBE_HIP(hipMemcpyAsync(&out, vz.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  return out;
 }
 
 }  // namespace cube

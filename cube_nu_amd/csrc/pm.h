@@ -21,6 +21,10 @@ struct PMLevel {
   int xdiv;          // 1: divide by gridsize (not a power of two), 0: multiply by xmul
   int nforce;        // pm%nforce: physical force cells 1..nforce
   int nte, ncb, nt;  // tile geometry for rhoc indexing
+  // kick coefficient in device memory, read in place of the by-value one when
+  // set (a replayed graph keeps its launch arguments; the coefficient changes
+  // every step)
+  const double* kdev = nullptr;
 };
 
 // One box: a tile (PM1, PM2) or a sub-tile (PM3).
@@ -98,6 +102,10 @@ uint64_t device_checksum(const void* p, size_t bytes, void* scratch8, hipStream_
 template <typename R>
 void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const DepBins& S, R* mesh,
                       float mass, hipStream_t s, int threads = 512);
+
+// a[i] += b[i] (the density of a second species added to a level's meshes).
+template <typename R>
+void mesh_add(R* a, const R* b, int64_t n, hipStream_t s);
 
 // Exclusive scan of int32 counts into int64 offsets (n+1 outputs).
 void scan_counts(const int32_t* in, int64_t* out, int64_t n, void* tmp, size_t tmp_bytes, hipStream_t s);

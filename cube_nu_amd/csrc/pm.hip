@@ -113,6 +113,17 @@ __device__ inline double pm_xpos(const PMLevel& L, int c, double frac, int shift
 
 // Deposits one particle's 27 TSC weights (floor f, weights w) into the
 // block's LDS accumulator.
+//
+// The accumulator of a BD^3 block stores cell (x, y, z) at row (z, y) and
+// column x ^ ((5y + 9z) mod BD).  The queued particles of one wavefront come
+// from a few neighbouring coarse cells, so for one tap their cells differ
+// mostly in y and z; unswizzled, the 64-bit LDS bank of a cell would depend
+// on x alone and the lanes' atomics would pile onto a handful of banks.
+template <int BD>
+__device__ inline int acc_slot(int x, int y, int z) {
+  return (z * BD + y) * BD + (x ^ ((5 * y + 9 * z) & (BD - 1)));
+}
+
 template <typename R, typename Acc, int BD>
 __device__ inline void deposit_particle(Acc* acc, const int f[3], const R w0[3][3], const int mlo[3], const int mhi[3]) {
   // FP32: the z weights carry the 2^32 of the fixed point; scaling by a power
@@ -130,14 +141,16 @@ __device__ inline void deposit_particle(Acc* acc, const int f[3], const R w0[3][
       const int my = f[1] + i2;
       if (!inner && (my < mlo[1] || my > mhi[1])) continue;
       const R wyz = w[1][i2];
-      const int row = ((mz - mlo[2]) * BD + (my - mlo[1])) * BD - mlo[0];
+      const int ly = my - mlo[1], lz = mz - mlo[2];
+      const int row = (lz * BD + ly) * BD, sw = (5 * ly + 9 * lz) & (BD - 1);
       for (int i1 = 0; i1 < 3; ++i1) {
         const int mx = f[0] + i1;
         if (!inner && (mx < mlo[0] || mx > mhi[0])) continue;
         const R cw = (w[0][i1] * wyz) * w[2][i3];  // dx(1,i1)*dx(2,i2)*dx(3,i3)
+        const int slot = row + ((mx - mlo[0]) ^ sw);
         if constexpr (std::is_same<R, float>::value)
-          atomicAdd(&acc[row + mx], (unsigned long long)__float2uint_rn(cw));
-        else atomicAdd(&acc[row + mx], cw);
+          atomicAdd(&acc[slot], (unsigned long long)__float2uint_rn(cw));
+        else atomicAdd(&acc[slot], cw);
       }
     }
   }
@@ -282,9 +295,10 @@ __global__ void __launch_bounds__(256) k_deposit(PMLevel L, const PMBox* __restr
     const int lx = t % BD, ly = (t / BD) % BD, lz = t / (BD * BD);
     const int m[3] = {mlo[0] + lx, mlo[1] + ly, mlo[2] + lz};
     if (m[0] > mhi[0] || m[1] > mhi[1] || m[2] > mhi[2]) continue;
+    const Acc av = acc[acc_slot<BD>(lx, ly, lz)];
     R v;
-    if constexpr (std::is_same<R, float>::value) v = (float)((double)acc[t] * (1.0 / (double)kFixF));
-    else v = acc[t];
+    if constexpr (std::is_same<R, float>::value) v = (float)((double)av * (1.0 / (double)kFixF));
+    else v = av;
     int a[3];
     for (int d = 0; d < 3; ++d) a[d] = m[d] + B.org[d] - L.m0;
     mesh[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]] = v * (R)mass;
@@ -464,9 +478,10 @@ __global__ void __launch_bounds__(NT) k_deposit_lists(PMLevel L, const PMBox* __
     const int lx = t % BD, ly = (t / BD) % BD, lz = t / (BD * BD);
     const int m[3] = {mlo[0] + lx, mlo[1] + ly, mlo[2] + lz};
     if (m[0] > mhi[0] || m[1] > mhi[1] || m[2] > mhi[2]) continue;
+    const Acc av = acc[acc_slot<BD>(lx, ly, lz)];
     R v;
-    if constexpr (std::is_same<R, float>::value) v = (float)((double)acc[t] * (1.0 / (double)kFixF));
-    else v = acc[t];
+    if constexpr (std::is_same<R, float>::value) v = (float)((double)av * (1.0 / (double)kFixF));
+    else v = av;
     int a[3];
     for (int d = 0; d < 3; ++d) a[d] = m[d] + B.org[d] - L.m0;
     mesh[B.moff + ((int64_t)a[2] * L.n + a[1]) * L.n + a[0]] = v * (R)mass;
@@ -598,6 +613,7 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
   __shared__ R red[4];
   __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4];
   __shared__ int64_t gof[MAXC];
+  if (L.kdev) kc = *L.kdev;
   const int nblk = (L.nforce + BD - 1) / BD;
   int64_t b = blockIdx.x;
   const int bx = (int)(b % nblk);
@@ -799,6 +815,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
                                                       const int32_t* __restrict__ rhoc,
                                                       const int64_t* __restrict__ off, const R* __restrict__ phi,
                                                       double kc, PMOut* __restrict__ out) {
+  if (L.kdev) kc = *L.kdev;
   constexpr int FX = IBX + 2, FY = IBY + 2, FZ = IBZ + 2, FN = FX * FY * FZ;
   constexpr int PX = IBX + 6, PY = IBY + 6, PZ = IBZ + 6, PN = PX * PY * PZ;
   constexpr int NF = (FN + IT - 1) / IT, NPH = (PN + IT - 1) / IT;
@@ -1343,6 +1360,24 @@ uint64_t device_checksum(const void* p, size_t bytes, void* scratch8, hipStream_
   CUBE_HIP(hipStreamSynchronize(s));
   return v;
 }
+
+namespace {
+template <typename R>
+__global__ void k_mesh_add(R* __restrict__ a, const R* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = a[i] + b[i];
+}
+}  // namespace
+
+template <typename R>
+void mesh_add(R* a, const R* b, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>(ceil_div(n, 256), 65536);
+  hipLaunchKernelGGL(k_mesh_add<R>, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
+  CUBE_HIP(hipGetLastError());
+}
+template void mesh_add<float>(float*, const float*, int64_t, hipStream_t);
+template void mesh_add<double>(double*, const double*, int64_t, hipStream_t);
 
 void zero_async(void* p, size_t bytes, hipStream_t s) {
   if (bytes == 0) return;

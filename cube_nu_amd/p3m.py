@@ -102,7 +102,9 @@ ABI_FUNCTIONS = (
     "cube_p3m_buffer_v_resident", "cube_p3m_state_upload",
     "cube_p3m_state_download", "cube_p3m_kick_resident", "cube_p3m_drift_resident", "cube_p3m_buffer_resident",
     "cube_p3m_step_resident", "cube_p3m_make_ic", "cube_p3m_cicpower", "cube_p3m_synchronize",
-    "cube_p3m_timing_enable", "cube_p3m_timing_reset", "cube_p3m_timing_report", "cube_p3m_version",
+    "cube_p3m_kick_resident_nu", "cube_p3m_add_thermal_velocities",
+    "cube_p3m_graph_enable", "cube_p3m_timing_enable", "cube_p3m_timing_reset", "cube_p3m_timing_report",
+    "cube_p3m_version",
 )
 
 _lib = None
@@ -155,6 +157,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cube_p3m_make_ic.argtypes = [P, C.POINTER(ICParams), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     lib.cube_p3m_cicpower.argtypes = [P, C.c_int32, vp_, C.c_int32, vp_, C.POINTER(C.c_double)]
     lib.cube_p3m_synchronize.argtypes = [P]
+    lib.cube_p3m_kick_resident_nu.argtypes = [P, P, C.POINTER(KickIn), C.c_float, C.POINTER(KickOut),
+                                              C.POINTER(KickOut)]
+    lib.cube_p3m_add_thermal_velocities.argtypes = [P, C.c_double, C.c_uint64, C.POINTER(C.c_float)]
+    lib.cube_p3m_graph_enable.argtypes = [P, C.c_int32]
     lib.cube_p3m_timing_enable.argtypes = [P, C.c_int32]
     lib.cube_p3m_timing_reset.argtypes = [P]
     lib.cube_p3m_timing_report.argtypes = [P, C.c_char_p, i64]
@@ -369,6 +375,23 @@ class P3M:
         self._check(self.lib.cube_p3m_kick_resident(self.ctx, C.byref(kin), C.byref(out)))
         return out.as_dict()
 
+    def kick_resident_nu(self, nu: "P3M", *, a_mid, dt, nu_mass, mass_p_cdm=1.0, phase_mask=0):
+        """Kick with the PM-only second species resident in `nu`
+        (cube_p3m_kick_resident_nu); returns (outputs, second species' outputs)."""
+        kin = KickIn(a_mid, dt, mass_p_cdm, phase_mask)
+        out, out_nu = KickOut(), KickOut()
+        self._check(self.lib.cube_p3m_kick_resident_nu(self.ctx, nu.ctx, C.byref(kin), float(nu_mass),
+                                                       C.byref(out), C.byref(out_nu)))
+        return out.as_dict(), out_nu.as_dict()
+
+    def add_thermal_velocities(self, v_scale: float, seed: int) -> float:
+        """Fermi-Dirac thermal velocities (cube_p3m_add_thermal_velocities);
+        returns the largest |v_z| afterwards."""
+        vz = C.c_float()
+        self._check(self.lib.cube_p3m_add_thermal_velocities(self.ctx, float(v_scale), int(seed) & (2 ** 64 - 1),
+                                                             C.byref(vz)))
+        return vz.value
+
     def drift_resident(self, *, dt_old, dt, vz_max):
         din = DriftIn(dt_old, dt, vz_max)
         out = DriftOut()
@@ -423,6 +446,10 @@ class P3M:
 
     def synchronize(self):
         self._check(self.lib.cube_p3m_synchronize(self.ctx))
+
+    def graph(self, enable: bool = True):
+        """Graph mode for full resident kicks (cube_p3m_graph_enable)."""
+        self._check(self.lib.cube_p3m_graph_enable(self.ctx, int(enable)))
 
     def timing(self, enable: bool = True):
         self._check(self.lib.cube_p3m_timing_enable(self.ctx, int(enable)))

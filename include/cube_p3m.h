@@ -216,6 +216,40 @@ int cube_p3m_cicpower(cube_p3m_ctx* ctx, int32_t nw, float* xi, int32_t nbin,
 /* ---- measurement -------------------------------------------------------- */
 int cube_p3m_synchronize(cube_p3m_ctx* ctx);
 /* Enable HIP-event timing of every kernel launch on the context stream. */
+/* Kick with a second, PM-only species (neutrino particles) resident in
+ * another context of the same geometry, coordinates and precision (the
+ * buffered layout, after its own buffer step).  Both species deposit into
+ * the PM1 and PM2 meshes -- this context's particles with in->mass_p_cdm,
+ * the other's with nu_mass -- and both are kicked by PM1 and PM2; PM3 and PP
+ * act on this context's particles only.  *out_nu receives the second
+ * species' PM1/PM2 maxima, Delta-t criteria, vz_max (for its drift) and PM1
+ * density sum (testrho).  No reference counterpart: the reference represents
+ * massive neutrinos by scaling the CDM transfer functions with (1 - f_nu)
+ * (initialize.f90:316-334, neutrinos/power_nu.f90:346-498); with neutrino
+ * particles the contexts keep mass_nu = 0 (unscaled tf) and the masses carry
+ * f_nu. */
+int cube_p3m_kick_resident_nu(cube_p3m_ctx* ctx, cube_p3m_ctx* nu,
+                              const cube_kick_in* in, float nu_mass,
+                              cube_kick_out* out, cube_kick_out* out_nu);
+/* Add Fermi-Dirac thermal velocities to the resident particles (neutrino
+ * particles; no reference counterpart): each particle, keyed by (seed,
+ * particle ID) so that buffer copies agree, receives u * v_scale (code
+ * velocity units) in a uniformly random direction, u = pc/(k T_nu) drawn from
+ * u^2/(e^u + 1).  Use a seed other than the input maker's.  *vz_max receives
+ * the largest |v_z| afterwards (the next drift's bound).  Needs IDs. */
+int cube_p3m_add_thermal_velocities(cube_p3m_ctx* ctx, double v_scale,
+                                    uint64_t seed, float* vz_max);
+/* Graph mode for full resident kicks (and the kick + buffer_v of
+ * cube_p3m_step_resident): the kick's sizing pass (sub-tile classes, list
+ * counts and fills) runs as usual and its compute pass (deposits, FFTs,
+ * interpolation + kick, PP force, buffer_v) is captured into a HIP graph
+ * once and replayed while the launch signature (box counts per level and
+ * the buffer addresses) is unchanged; the kick coefficient is read from
+ * device memory.  Results are bit-identical to graph mode off.  Levels that
+ * need more than one workspace batch run uncaptured.  Counts of captures and
+ * launches appear in the timing report (graph_captures, graph_launches).
+ * No reference counterpart (the reference's step loop is main.f90:30-59). */
+int cube_p3m_graph_enable(cube_p3m_ctx* ctx, int32_t enable);
 int cube_p3m_timing_enable(cube_p3m_ctx* ctx, int32_t enable);
 int cube_p3m_timing_reset(cube_p3m_ctx* ctx);
 /* Text report, one line per kernel: "name calls total_ms bytes flops". */

@@ -202,6 +202,7 @@ struct cube_p3m_ctx {
   cube_p3m_ctx* nu = nullptr;
   float nu_mass = 0;
   DevBuf<char> work2;  // the second species' level meshes before they are added
+  DevBuf<char> ftune;  // the FFT shape trials' meshes (released after the trials)
   void sig_add(std::initializer_list<uint64_t> v) { gsig_acc.insert(gsig_acc.end(), v.begin(), v.end()); }
   void graph_drop() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -407,6 +408,25 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
       } else if (CP)
         pm_deposit<R>((int)g.izipx, L, boxes.p, nb, xpc(), rhocc(), off.p, mesh, in.mass_p_cdm, s);
     }
+    // launch shape of the tile / sub-tile FFT passes, tuned on a fixed
+    // pattern in scratch meshes (every shape gives the same result)
+    int fshape = 0;
+    if (std::is_same<R, float>::value && level >= 2 && fft_plane_tunable((int)n)) {
+      const std::string key = tune_key("fft_shape", level, n);
+      const int64_t nt = std::min<int64_t>(nmesh, 256);
+      const bool trials = !tune_picks.count(key);
+      if (trials) ftune.ensure(sizeof(R) * (size_t)(nt * nreal));
+      const R n3t = (R)((float)n * (float)n * (float)n);
+      fshape = autotune(
+          key, {0, 1, 2},
+          [&](int sh) {
+            fill_pattern((float*)ftune.p, nt * nreal, s);
+            fft_convolve<R>(fc, (R*)ftune.p, cplx, G, nt, n3t, (R)0, s, nullptr, "", sh);
+          },
+          ftune.p, sizeof(R) * (size_t)(nt * nreal));
+      if (trials) ftune.release();
+      if (SZ) sig_add({(uint64_t)fshape});
+    }
     if (!CP) continue;
     if (nu && level <= 2) {
       KScope k(&tm, (t + "deposit_nu").c_str(), cells * sizeof(R));
@@ -426,7 +446,7 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
     }
     const R n3 = (R)((float)n * (float)n * (float)n);
     if (level == 1) fft_convolve<R>(fc, mesh, cplx, G, 1, (R)0, (R)(float)n, s, &tm, tag);  // pencil_fft.f90:77
-    else fft_convolve<R>(fc, mesh, cplx, G, nmesh, n3, (R)0, s, &tm, tag);                // kick.f90:846
+    else fft_convolve<R>(fc, mesh, cplx, G, nmesh, n3, (R)0, s, &tm, tag, fshape);        // kick.f90:846
     {
       KScope k(&tm, (t + "interp_kick").c_str(), cells * sizeof(R));
       pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, kick_coef(in), o, s);

@@ -32,15 +32,19 @@ struct FFTCube {
 // rho/phi: [box][z][y][x] real (n^3 per box); work: complex scratch of
 // nbox * n*n*(n/2+1) elements.  The division follows the reference: one FP
 // division by div3 when div3 != 0, else three divisions by div1.
+// shape: launch shape of the size-specialised passes (0, 1 or 2; the same
+// result for each), picked by the caller's autotuner.
 template <typename R>
 void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox, R div3,
-                  R div1, hipStream_t s, KTimer* tm = nullptr, const char* tag = "");
+                  R div1, hipStream_t s, KTimer* tm = nullptr, const char* tag = "", int shape = 0);
 
 // Size-specialised 3-pass form (fft_plane.hip); returns false when n has no
 // specialisation.
 template <typename R>
 bool fft_convolve_plane(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox,
-                        R div3, R div1, hipStream_t s, KTimer* tm, const char* tag);
+                        R div3, R div1, hipStream_t s, KTimer* tm, const char* tag, int shape = 0);
+// True when n has a size-specialised form with several launch shapes (FP32).
+bool fft_plane_tunable(int n);
 
 // Forward-only and inverse-only pieces (used by the input maker and tests).
 template <typename R>

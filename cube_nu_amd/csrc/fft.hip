@@ -373,11 +373,20 @@ void fft_c2r3d(const FFTCube<R>& c, typename Cplx<R>::T* in, R* out, int64_t nbo
   CUBE_HIP(hipGetLastError());
 }
 
+bool fft_plane_tunable(int n) {
+  if (!use_plane_path()) return false;
+  switch (n) {
+    case 64: case 72: case 80: case 96: case 128: case 144: case 160: return true;
+    default: return false;
+  }
+}
+
 template <typename R>
 void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox, R div3,
-                  R div1, hipStream_t s, KTimer* tm, const char* tag) {
+                  R div1, hipStream_t s, KTimer* tm, const char* tag, int shape) {
   using C = typename Cplx<R>::T;
-  if (use_plane_path() && fft_convolve_plane<R>(c, rho, work, G, nbox, div3, div1, s, tm ? tm : &null_timer(), tag))
+  if (use_plane_path() &&
+      fft_convolve_plane<R>(c, rho, work, G, nbox, div3, div1, s, tm ? tm : &null_timer(), tag, shape))
     return;
   const int n = c.n, h = n / 2;
   const int64_t nrows = nbox * n * n;
@@ -416,9 +425,9 @@ void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const 
 template struct FFTCube<float>;
 template struct FFTCube<double>;
 template void fft_convolve<float>(const FFTCube<float>&, float*, float2*, const float*, int64_t, float, float,
-                                  hipStream_t, KTimer*, const char*);
+                                  hipStream_t, KTimer*, const char*, int);
 template void fft_convolve<double>(const FFTCube<double>&, double*, double2*, const float*, int64_t, double, double,
-                                   hipStream_t, KTimer*, const char*);
+                                   hipStream_t, KTimer*, const char*, int);
 template void fft_r2c3d<float>(const FFTCube<float>&, const float*, float2*, int64_t, hipStream_t);
 template void fft_r2c3d<double>(const FFTCube<double>&, const double*, double2*, int64_t, hipStream_t);
 template void fft_c2r3d<float>(const FFTCube<float>&, float2*, float*, int64_t, hipStream_t);

@@ -303,12 +303,9 @@ __global__ void __launch_bounds__(NT) k_plane_inv(const CP<R>* __restrict__ in, 
   }
 }
 
-template <typename R, int N>
-bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t nbox, R div3, R div1,
-               hipStream_t s, KTimer* tm, const std::string& t) {
-  // threads per plane / slice (measured at n = 80 and 160 on MI355X)
-  constexpr int NT = N >= 128 ? 1024 : (N >= 80 ? 512 : 256);
-  constexpr int NTZ = N >= 80 ? 1024 : 256;
+template <typename R, int N, int NT, int NTZ, int YB>
+void run_plane_shape(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t nbox, R div3, R div1,
+                     hipStream_t s, KTimer* tm, const std::string& t) {
   const double Gb = (double)N * N * N * sizeof(R) * nbox;
   const double Kb = (double)N * N * (N / 2 + 1) * sizeof(CP<R>) * nbox;
   const double Gk = (double)N * N * (N / 2 + 1) * sizeof(float);
@@ -319,8 +316,6 @@ bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t
   }
   {
     KScope k(tm, (t + "fft_z_green").c_str(), 2 * Kb + Gk);
-    // two slices per block where their LDS fits in 64 KB
-    constexpr int YB = (N % 2 == 0 && 2 * N * (N / 2 + 1) * (int)sizeof(CP<R>) <= 65536) ? 2 : 1;
     hipLaunchKernelGGL((k_zslice<R, N, NTZ, YB>), dim3((unsigned)(nbox * (N / YB))), dim3(NTZ), 0, s, work, G, c.tw);
     CUBE_HIP(hipGetLastError());
   }
@@ -330,6 +325,34 @@ bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t
                        div1);
     CUBE_HIP(hipGetLastError());
   }
+}
+
+// Launch shapes: shape 0 = threads per plane / slice measured best at n = 80 and
+// 160 on MI355X; shape 1 = the other thread count for the plane passes; shape
+// 2 = one z slice per workgroup instead of two (or half the threads where one
+// slice already fills the workgroup's LDS).  Every shape computes each
+// butterfly with the same operations, so all give the same result; the FP64
+// form keeps shape 0.
+template <typename R, int N>
+bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t nbox, R div3, R div1,
+               hipStream_t s, KTimer* tm, const std::string& t, int shape) {
+  constexpr int NT = N >= 128 ? 1024 : (N >= 80 ? 512 : 256);
+  constexpr int NTZ = N >= 80 ? 1024 : 256;
+  // two slices per block where their LDS fits in 64 KB
+  constexpr int YB = (N % 2 == 0 && 2 * N * (N / 2 + 1) * (int)sizeof(CP<R>) <= 65536) ? 2 : 1;
+  if constexpr (std::is_same<R, float>::value && N >= 64) {
+    constexpr int NT1 = N >= 128 ? 512 : (N >= 80 ? 256 : 512);
+    constexpr int NTZ2 = YB == 2 ? NTZ : (NTZ >= 512 ? NTZ / 2 : 512);
+    if (shape == 1) {
+      run_plane_shape<R, N, NT1, NTZ, YB>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+      return true;
+    }
+    if (shape == 2) {
+      run_plane_shape<R, N, NT, NTZ2, 1>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+      return true;
+    }
+  }
+  run_plane_shape<R, N, NT, NTZ, YB>(c, rho, work, G, nbox, div3, div1, s, tm, t);
   return true;
 }
 
@@ -339,31 +362,31 @@ bool run_plane(const FFTCube<R>& c, R* rho, CP<R>* work, const float* G, int64_t
 // generic line passes).
 template <typename R>
 bool fft_convolve_plane(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox,
-                        R div3, R div1, hipStream_t s, KTimer* tm, const char* tag) {
+                        R div3, R div1, hipStream_t s, KTimer* tm, const char* tag, int shape) {
   std::string t(tag);
   switch (c.n) {
-    case 16: return run_plane<R, 16>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 24: return run_plane<R, 24>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 32: return run_plane<R, 32>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 48: return run_plane<R, 48>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 64: return run_plane<R, 64>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 72: return run_plane<R, 72>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 80: return run_plane<R, 80>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    case 96: return run_plane<R, 96>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    case 16: return run_plane<R, 16>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 24: return run_plane<R, 24>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 32: return run_plane<R, 32>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 48: return run_plane<R, 48>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 64: return run_plane<R, 64>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 72: return run_plane<R, 72>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 80: return run_plane<R, 80>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    case 96: return run_plane<R, 96>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
     default: break;
   }
   // n^2 (n/2+1) planes of FP64 beyond 96 do not fit in LDS
   if constexpr (std::is_same<R, float>::value) {
-    if (c.n == 128) return run_plane<R, 128>(c, rho, work, G, nbox, div3, div1, s, tm, t);
-    if (c.n == 144) return run_plane<R, 144>(c, rho, work, G, nbox, div3, div1, s, tm, t);  // PM3 at 1024^3
-    if (c.n == 160) return run_plane<R, 160>(c, rho, work, G, nbox, div3, div1, s, tm, t);
+    if (c.n == 128) return run_plane<R, 128>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
+    if (c.n == 144) return run_plane<R, 144>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);  // PM3 at 1024^3
+    if (c.n == 160) return run_plane<R, 160>(c, rho, work, G, nbox, div3, div1, s, tm, t, shape);
   }
   return false;
 }
 
 template bool fft_convolve_plane<float>(const FFTCube<float>&, float*, float2*, const float*, int64_t, float, float,
-                                        hipStream_t, KTimer*, const char*);
+                                        hipStream_t, KTimer*, const char*, int);
 template bool fft_convolve_plane<double>(const FFTCube<double>&, double*, double2*, const float*, int64_t, double,
-                                         double, hipStream_t, KTimer*, const char*);
+                                         double, hipStream_t, KTimer*, const char*, int);
 
 }  // namespace cube

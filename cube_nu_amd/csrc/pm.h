@@ -103,6 +103,9 @@ template <typename R>
 void pm_deposit_lists(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const DepBins& S, R* mesh,
                       float mass, hipStream_t s, int threads = 512);
 
+// A fixed pseudo-random pattern in [-1/2, 1/2) (the autotuner's FFT input).
+void fill_pattern(float* p, int64_t n, hipStream_t s);
+
 // a[i] += b[i] (the density of a second species added to a level's meshes).
 template <typename R>
 void mesh_add(R* a, const R* b, int64_t n, hipStream_t s);

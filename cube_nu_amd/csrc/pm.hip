@@ -1369,6 +1369,24 @@ __global__ void k_mesh_add(R* __restrict__ a, const R* __restrict__ b, int64_t n
 }
 }  // namespace
 
+namespace {
+__global__ void k_fill_pattern(float* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = (uint64_t)i * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 31)) * 0xbf58476d1ce4e5b9ull;
+    z ^= z >> 29;
+    p[i] = (float)(z >> 40) * (1.0f / 16777216.0f) - 0.5f;
+  }
+}
+}  // namespace
+
+void fill_pattern(float* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>(ceil_div(n, 256), 65536);
+  hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+  CUBE_HIP(hipGetLastError());
+}
+
 template <typename R>
 void mesh_add(R* a, const R* b, int64_t n, hipStream_t s) {
   if (n <= 0) return;

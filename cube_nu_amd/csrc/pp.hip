@@ -247,7 +247,11 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // padded by kPPPad >= U entries so the reads need no clamp).  A two-pass form
 // (in-range tests of 32 candidates into a bit mask, then the pair law over
 // the set bits only, each partner read again) measured 37.9 against 27.2 ms
-// at 512^3: the second pass's reads come from L2, one per pair.
+// at 512^3: the second pass's reads come from L2, one per pair.  Staging
+// each brick's neighbourhood (up to 2560 entries) in LDS and reading the
+// candidates from there measured 50.3 ms: 50 KB of LDS per workgroup leaves
+// three wavefronts per SIMD, and the lanes' scattered 16-byte LDS reads
+// conflict.
 // FP32 form: eight 256-thread workgroups per CU (the launch bound's second
 // value), i.e. at most 64 VGPRs and eight wavefronts per SIMD to cover the
 // candidate loads' latency (28.5 vs 30.2 ms at 512^3 with 71 VGPRs and seven

@@ -122,7 +122,10 @@ def measured_traffic(kernel, ng):
 
 
 def roofline(rep, steps, ng, pairs_total=None):
-    kern = {k: v for k, v in rep.items() if v["ms"] > 0 and k != "green"}
+    # timed kernels of the steps (not the input's Green's functions, nor the
+    # autotuner's trials, which the report lists as autotune:<key>:<candidate>)
+    kern = {k: v for k, v in rep.items()
+            if v["ms"] > 0 and v["calls"] > 0 and k != "green" and not k.startswith("autotune:")}
     if not kern:
         return None, {}
     name = max(kern, key=lambda k: kern[k]["ms"])

@@ -91,6 +91,47 @@ def run_form(ng, precision, steps, warmup, z, box, timing=True, per_tile=False, 
             "pairs": pairs, "graph": gl, "digest": h}
 
 
+def swap_in_rate(ng, z, box, calls=3):
+    """The swap-in boundary (cube_p3m_kick: the caller's host arrays copied to
+    the device, the kick, the velocities copied back) against the resident
+    kick, on the bench's input: the PCIe-inclusive rate of the kick."""
+    import numpy as np
+    from cube_nu_amd import P3M
+
+    p = P3M(ng, izipx=2, box=box)
+    try:
+        p.make_ic(seed=20260101, z=z, nic=2)
+        a = 1.0 / (1.0 + z)
+        k0 = p.kick_resident(a_mid=a, dt=0.0)
+        dt = min(k0["dt_pm1"], k0["dt_pm2"], k0["dt_pm3"], k0["dt_pp"], 1.0)
+        xp, vp, pid, rhoc = p.download()
+        v = vp.copy()
+        p.kick(xp, v, rhoc, a_mid=a, dt=dt)  # warm-up (the arrays' first touch, growth of the buffers)
+        t_swap = []
+        for _ in range(calls):
+            v = vp.copy()
+            t0 = time.perf_counter()
+            p.kick(xp, v, rhoc, a_mid=a, dt=dt)
+            t_swap.append(time.perf_counter() - t0)
+        p.upload(xp, vp, pid, rhoc)
+        p.kick_resident(a_mid=a, dt=dt)
+        t_res = []
+        for _ in range(calls):
+            p.synchronize()
+            t0 = time.perf_counter()
+            p.kick_resident(a_mid=a, dt=dt)
+            t_res.append(time.perf_counter() - t0)
+    finally:
+        p.close()
+    n = xp.shape[0]
+    moved = xp.nbytes + vp.nbytes * 2 + rhoc.nbytes
+    ts, tr = statistics.median(t_swap), statistics.median(t_res)
+    return {"particles": n, "host_bytes_moved_per_kick": moved,
+            "kick_ms_swap_in": 1e3 * ts, "kick_ms_resident": 1e3 * tr,
+            "particle_updates_per_s_swap_in": n / ts, "particle_updates_per_s_resident": n / tr,
+            "note": "extended (buffered) particle count; swap-in = pageable host arrays through cube_p3m_kick"}
+
+
 def autotune_picks(rep):
     """{choice: {"pick": c, "ms": {candidate: trial ms}}} from the timing
     report's autotune lines."""
@@ -501,6 +542,8 @@ def main():
     ap.add_argument("--z", type=float, default=2.0)
     ap.add_argument("--box", type=float, default=600.0)
     ap.add_argument("--graph", type=int, default=1, help="0 = skip the graph-on leg of the step")
+    ap.add_argument("--swap-in", type=int, default=1,
+                    help="0 = skip the PCIe-inclusive kick through host arrays (cube_p3m_kick)")
     ap.add_argument("--nic", type=int, default=2, help="potential grid refinement of the input maker (ic.f90 nic)")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="steps of the FP64 forms, batched and plain per-tile (0 = skip)")
@@ -588,6 +631,12 @@ def main():
             prec = precision_study(args.ng, zp, args.box)
         except Exception as e:  # reported, not hidden
             prec = {"error": str(e)}
+    swap = None
+    if args.swap_in:
+        try:
+            swap = swap_in_rate(args.ng, args.z, args.box)
+        except Exception as e:  # reported, not hidden
+            swap = {"error": str(e)}
     nu = None
     if args.nu_study:
         try:
@@ -611,7 +660,7 @@ def main():
                    "ng": args.ng, "izipx": 2, "dt": r32["dt"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64_form": fp64, "fp64_plain_per_tile": fp64_plain,
         "phase_ms_per_step": phases, "autotune": r32["autotune"], "pm_forward_adjoint": pm_adj,
-        "fp32_vs_fp64": prec, "cdm_nu_fp32_vs_fp64": nu, "graph": graph,
+        "fp32_vs_fp64": prec, "cdm_nu_fp32_vs_fp64": nu, "graph": graph, "swap_in": swap,
     }
     print(json.dumps(line), flush=True)
 

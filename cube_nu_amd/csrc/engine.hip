@@ -189,6 +189,16 @@ struct cube_p3m_ctx {
   // compute pass reads -- stays the same.  The kick coefficient, the one
   // argument that changes every step, is read from device memory (kcoef).
   bool graph_on = false;
+  // Overlapped kick (full resident kicks after a context's first, graph mode
+  // off): the PM3 and PP sizing runs on a second stream while PM1 and PM2
+  // compute on the first; the first stream waits for the sizing before PM3.
+  bool overlap_on = true;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_sized = nullptr;
+  int64_t kicks_done = 0;
+  bool classified = false;  // kick_pass: the classes are already on the host
+  bool skip_zero = false;   // kick_pass: the reductions are already cleared
+  bool no_trials = false;   // autotune: keep the first candidate, untimed
   int kph = 0;           // 0 both passes in one, 1 sizing only, 2 compute only
   bool graph_ok = true;  // cleared by a sizing pass that needs more than one batch per level
   std::vector<uint64_t> gsig_acc, gsig;
@@ -212,6 +222,8 @@ struct cube_p3m_ctx {
 
   ~cube_p3m_ctx() {
     graph_drop();
+    if (ev_sized) (void)hipEventDestroy(ev_sized);
+    if (s2) (void)hipStreamDestroy(s2);
     for (int i = 0; i < 7; ++i) {
       fcf[i].release();
       fcd[i].release();
@@ -270,6 +282,9 @@ struct cube_p3m_ctx {
   void kick_pass(const cube_kick_in& in, cube_kick_out& out, float* density_host, int mask, bool with_buffer_v,
                  std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h);
   template <typename R>
+  void kick_overlapped(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v, std::vector<int32_t>& ires_h,
+                       std::vector<int32_t>& cmax_h);
+  template <typename R>
   void run_level(int level, int slot, const std::vector<PMBox>& bx, const float* G, const FFTCube<R>& fc,
                  const PMLevel& L, const cube_kick_in& in, PMOut* o, const char* tag, float* density_host = nullptr);
   void pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h, const cube_kick_in& in,
@@ -301,6 +316,9 @@ void cube_p3m_ctx::init() {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(CUBE_ERR_NO_DEVICE, "no HIP device visible");
   CUBE_HIP(hipGetDevice(&device));
   CUBE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  CUBE_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  CUBE_HIP(hipEventCreateWithFlags(&ev_sized, hipEventDisableTiming));
+  if (const char* e = std::getenv("CUBE_KICK_OVERLAP")) overlap_on = !(e[0] == '0');
   tm.stream = s;
   const int sizes[7] = {0, (int)g.nc, (int)g.ngt, (int)g.nft[2], (int)g.nft[3], (int)g.nft[4], (int)g.nft[5]};
   for (int l = 1; l <= 6; ++l) {
@@ -576,6 +594,10 @@ int cube_p3m_ctx::autotune(const std::string& key, const std::vector<int>& cand,
                            size_t result_bytes) {
   auto known = tune_picks.find(key);
   if (known != tune_picks.end()) return known->second;
+  // no trials while another stream computes (they would share its scratch and
+  // see its velocities change): the first candidate now, the trials at the
+  // next use on the first stream
+  if (no_trials) return cand[0];
   int pick = 0;
   struct Ev {  // destroyed on every exit, errors included
     hipEvent_t e = nullptr;
@@ -638,7 +660,7 @@ template <typename R>
 void cube_p3m_ctx::kick_pass(const cube_kick_in& in, cube_kick_out& out, float* density_host, int mask,
                              bool with_buffer_v, std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h) {
   const bool SZ = kph != 2, CP = kph != 1;
-  if (CP) {
+  if (CP && !skip_zero) {
     zero_async(pmout.p, sizeof(PMOut) * 4, s);
     zero_async(ppout.p, sizeof(PPOut), s);
     if (nu) zero_async(nu->pmout.p, sizeof(PMOut) * 4, s);
@@ -684,7 +706,7 @@ void cube_p3m_ctx::kick_pass(const cube_kick_in& in, cube_kick_out& out, float* 
   }
   // classification (kick.f90:108-138); a compute-only pass reuses the sizing
   // pass's classes
-  if ((mask & 12) && SZ) {
+  if ((mask & 12) && SZ && !classified) {
     classify_subtiles(g, cfg.ires_fixed, rhocc(), ires.p, napm.p, cmax.p, s);
     CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
     CUBE_HIP(hipMemcpyAsync(cmax_h.data(), cmax.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
@@ -728,6 +750,75 @@ void cube_p3m_ctx::kick_pass(const cube_kick_in& in, cube_kick_out& out, float* 
   }
 }
 
+// Full kick with the PM3/PP sizing on the second stream, overlapped with the
+// PM1 and PM2 levels on the first (only the host-visible order changes: the
+// same kernels on the same data, so the result is the same).  The sizing
+// reads positions and counts only and writes the levels' own list buffers;
+// the workspace meshes are grown for every level before anything is queued,
+// so no buffer the first stream uses is reallocated while it runs.
+template <typename R>
+void cube_p3m_ctx::kick_overlapped(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v,
+                                   std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h) {
+  using C = typename Cplx<R>::T;
+  struct Reset {  // one-pass kicks on the first stream again on every exit, errors included
+    cube_p3m_ctx* c;
+    hipStream_t s0;
+    bool timing;
+    ~Reset() {
+      c->s = s0;
+      c->kph = 0;
+      c->classified = false;
+      c->skip_zero = false;
+      c->no_trials = false;
+      c->tm.enabled = timing;
+    }
+  } reset{this, s, tm.enabled};
+  // 1. classes (one read-back) and the largest level workspace
+  classify_subtiles(g, cfg.ires_fixed, rhocc(), ires.p, napm.p, cmax.p, s);
+  CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipMemcpyAsync(cmax_h.data(), cmax.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipMemcpyAsync(out.napm, napm.p, sizeof(int32_t) * 7, hipMemcpyDeviceToHost, s));
+  CUBE_HIP(hipStreamSynchronize(s));
+  classified = true;
+  {
+    int64_t need = 0;
+    auto level_ws = [&](int64_t n, int64_t nmesh) {
+      if (nmesh <= 0) return;
+      const int64_t per = n * n * n * (int64_t)sizeof(R) + n * n * (n / 2 + 1) * (int64_t)sizeof(C);
+      const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nmesh, ws_budget() / per));
+      need = std::max(need, batch * per);
+    };
+    level_ws(g.nc, 1);
+    level_ws(g.ngt, g.ntiles);
+    for (int iapm = 2; iapm <= 5; ++iapm)
+      level_ws(g.nft[iapm], (int64_t)std::count(ires_h.begin(), ires_h.end(), iapm));
+    work.ensure((size_t)need);
+  }
+  // 2. PM1 and PM2 on the first stream
+  kph = 0;
+  kick_pass<R>(in, out, nullptr, 3, false, ires_h, cmax_h);
+  skip_zero = true;
+  // 3. PM3 and PP sizing on the second stream (untimed: the timer records on
+  //    the first stream)
+  tm.enabled = false;
+  const hipStream_t s0 = s;
+  s = s2;
+  kph = 1;
+  graph_ok = true;
+  gsig_acc.clear();
+  no_trials = true;
+  kick_pass<R>(in, out, nullptr, 12, false, ires_h, cmax_h);
+  no_trials = false;
+  CUBE_HIP(hipEventRecord(ev_sized, s2));
+  s = s0;
+  tm.enabled = reset.timing;
+  CUBE_HIP(hipStreamWaitEvent(s, ev_sized, 0));
+  // 4. PM3 and PP compute (a level needing several workspace batches sizes
+  //    and computes again in one pass: its batches share one set of lists)
+  kph = graph_ok ? 2 : 0;
+  kick_pass<R>(in, out, nullptr, 12, with_buffer_v, ires_h, cmax_h);
+}
+
 template <typename R>
 void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* density_host, bool with_buffer_v) {
   if (layout != 1) throw Error(CUBE_ERR_STATE, "kick needs the buffered (extended) particle layout");
@@ -742,7 +833,12 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
   wsb = -1;
   wsb = ws_budget();
   const bool use_graph = graph_on && mask == 15 && !density_host && !nu;
-  if (!use_graph) {
+  const bool overlap =
+      !use_graph && overlap_on && mask == 15 && !density_host && !nu && cfg.pm3 && cfg.pp && kicks_done > 0;
+  ++kicks_done;
+  if (overlap) {
+    kick_overlapped<R>(in, out, with_buffer_v, ires_h, cmax_h);
+  } else if (!use_graph) {
     kph = 0;
     kick_pass<R>(in, out, density_host, mask, with_buffer_v, ires_h, cmax_h);
   } else {

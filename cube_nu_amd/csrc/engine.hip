@@ -780,19 +780,33 @@ void cube_p3m_ctx::kick_overlapped(const cube_kick_in& in, cube_kick_out& out, b
   CUBE_HIP(hipMemcpyAsync(out.napm, napm.p, sizeof(int32_t) * 7, hipMemcpyDeviceToHost, s));
   CUBE_HIP(hipStreamSynchronize(s));
   classified = true;
+  bool multi = false;  // a PM3 or PP level in several workspace batches
   {
     int64_t need = 0;
-    auto level_ws = [&](int64_t n, int64_t nmesh) {
+    auto level_ws = [&](int64_t n, int64_t nmesh, bool sub) {
       if (nmesh <= 0) return;
       const int64_t per = n * n * n * (int64_t)sizeof(R) + n * n * (n / 2 + 1) * (int64_t)sizeof(C);
       const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(nmesh, ws_budget() / per));
       need = std::max(need, batch * per);
+      if (sub && batch < nmesh) multi = true;
     };
-    level_ws(g.nc, 1);
-    level_ws(g.ngt, g.ntiles);
-    for (int iapm = 2; iapm <= 5; ++iapm)
-      level_ws(g.nft[iapm], (int64_t)std::count(ires_h.begin(), ires_h.end(), iapm));
+    level_ws(g.nc, 1, false);
+    level_ws(g.ngt, g.ntiles, false);
+    for (int iapm = 2; iapm <= 5; ++iapm) {
+      const int64_t nb = (int64_t)std::count(ires_h.begin(), ires_h.end(), iapm);
+      level_ws(g.nft[iapm], nb, true);
+      // the PP batch rule of pp_phase
+      const int64_t m = (int64_t)g.ntt * kRatioSF[iapm] + 2, m3 = m * m * m * kSX;
+      if (nb > 0 && std::max<int64_t>(1, ws_budget() / 4 / (m3 * 12)) < nb) multi = true;
+    }
     work.ensure((size_t)need);
+  }
+  if (multi) {  // lists per batch: PM1/PM2, then PM3/PP, in one pass each on the first stream
+    kph = 0;
+    kick_pass<R>(in, out, nullptr, 3, false, ires_h, cmax_h);
+    skip_zero = true;
+    kick_pass<R>(in, out, nullptr, 12, with_buffer_v, ires_h, cmax_h);
+    return;
   }
   // 2. PM1 and PM2 on the first stream
   kph = 0;

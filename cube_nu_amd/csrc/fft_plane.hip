@@ -18,6 +18,9 @@ struct Fac {
 constexpr Fac factor(int n) {
   Fac f{};
   int m = n;
+  // the largest in-register radices first: fewer LDS round trips per line
+  while (m % 16 == 0) { f.r[f.n++] = 16; m /= 16; }
+  while (m % 8 == 0) { f.r[f.n++] = 8; m /= 8; }
   while (m % 4 == 0) { f.r[f.n++] = 4; m /= 4; }
   while (m % 2 == 0) { f.r[f.n++] = 2; m /= 2; }
   while (m % 3 == 0) { f.r[f.n++] = 3; m /= 3; }
@@ -51,10 +54,75 @@ template <typename C>
 __device__ inline C cj(C a) { return CT<C>::mk(a.x, -a.y); }
 
 template <int SIGN, int R, typename C>
+__device__ inline void bfly(C* v);
+
+// 4-point DFT of (a, b, c, d) in place, e^{SIGN 2 pi i jk/4}
+template <int SIGN, typename C>
+__device__ inline void dft4(C& a, C& b, C& c, C& d) {
+  using RR = typename CT<C>::R;
+  const RR s = (RR)SIGN;
+  const C p = add(a, c), q = sub(a, c), r = add(b, d), t = miu(sub(b, d), s);
+  a = add(p, r);
+  c = sub(p, r);
+  b = add(q, t);
+  d = sub(q, t);
+}
+
+template <int SIGN, int R, typename C>
 __device__ inline void bfly(C* v) {
   using RR = typename CT<C>::R;
   const RR s = (RR)SIGN;
-  if constexpr (R == 2) {
+  if constexpr (R == 16) {
+    // four 4-point DFTs over n = r + 4m, twiddles W16^(r k1), then 4-point
+    // DFTs over r: X[k1 + 4 k2]
+    C u[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C a = v[r], b = v[r + 4], c = v[r + 8], d = v[r + 12];
+      dft4<SIGN>(a, b, c, d);
+      u[4 * r + 0] = a;
+      u[4 * r + 1] = b;
+      u[4 * r + 2] = c;
+      u[4 * r + 3] = d;
+    }
+    constexpr double kC[10] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+                               -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613, -1.0,
+                               -0.92387953251128675613};
+    constexpr double kS[10] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613, 1.0,
+                               0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+                               -0.38268343236508977173};
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C t[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = r * k1;  // 0..9
+        const C w = CT<C>::mk((RR)kC[j], s * (RR)kS[j]);
+        t[r] = (j == 0) ? u[4 * r + k1] : mul(u[4 * r + k1], w);
+      }
+      dft4<SIGN>(t[0], t[1], t[2], t[3]);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  } else if constexpr (R == 8) {
+    // even/odd 4-point DFTs, X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k]
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<SIGN>(e0, e1, e2, e3);
+    dft4<SIGN>(o0, o1, o2, o3);
+    const RR h = (RR)0.70710678118654752440;
+    const C w1 = CT<C>::mk(h, s * h), w3 = CT<C>::mk(-h, s * h);
+    o1 = mul(o1, w1);
+    o2 = miu(o2, s);
+    o3 = mul(o3, w3);
+    v[0] = add(e0, o0);
+    v[4] = sub(e0, o0);
+    v[1] = add(e1, o1);
+    v[5] = sub(e1, o1);
+    v[2] = add(e2, o2);
+    v[6] = sub(e2, o2);
+    v[3] = add(e3, o3);
+    v[7] = sub(e3, o3);
+  } else if constexpr (R == 2) {
     C a = v[0], b = v[1];
     v[0] = add(a, b);
     v[1] = sub(a, b);

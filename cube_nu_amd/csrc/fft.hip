@@ -16,6 +16,9 @@ FFTStages make_stages(int n) {
   s.n = n;
   s.nst = 0;
   int m = n;
+  // the largest in-register radices first: fewer LDS round trips per line
+  while (m % 16 == 0) { s.rad[s.nst++] = 16; m /= 16; }
+  while (m % 8 == 0) { s.rad[s.nst++] = 8; m /= 8; }
   while (m % 4 == 0) { s.rad[s.nst++] = 4; m /= 4; }
   while (m % 2 == 0) { s.rad[s.nst++] = 2; m /= 2; }
   while (m % 3 == 0) { s.rad[s.nst++] = 3; m /= 3; }
@@ -119,30 +122,131 @@ __device__ inline void dft(int R, C* v, const C* tw, int twn, int twstride) {
   }
 }
 
+// 4-point DFT of (a, b, c, d) in place, e^{SIGN 2 pi i jk/4}.
+template <int SIGN, typename C>
+__device__ inline void dft4(C& a, C& b, C& c, C& d) {
+  using RR = typename CO<C>::R;
+  const RR s = (RR)SIGN;
+  const C p = cadd(a, c), q = csub(a, c), r = cadd(b, d), t = cmul_iu(csub(b, d), s);
+  a = cadd(p, r);
+  c = csub(p, r);
+  b = cadd(q, t);
+  d = csub(q, t);
+}
+
+// Compile-time radix R in {2, 3, 4, 5, 8, 16}: the same butterflies as dft()
+// for 2..5, register-resident (a runtime radix indexes its array and puts it
+// in scratch memory).
+template <int SIGN, int R, typename C>
+__device__ inline void dft_fixed(C* v, const C* tw, int twn, int twstride) {
+  using RR = typename CO<C>::R;
+  const RR s = (RR)SIGN;
+  if constexpr (R == 16) {
+    // 4-point DFTs over n = r + 4m, twiddles W16^(r k1), 4-point DFTs over r
+    C u[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      C a = v[r], b = v[r + 4], c = v[r + 8], d = v[r + 12];
+      dft4<SIGN>(a, b, c, d);
+      u[4 * r + 0] = a;
+      u[4 * r + 1] = b;
+      u[4 * r + 2] = c;
+      u[4 * r + 3] = d;
+    }
+    constexpr double kC[10] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+                               -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613, -1.0,
+                               -0.92387953251128675613};
+    constexpr double kS[10] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613, 1.0,
+                               0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+                               -0.38268343236508977173};
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C t[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = r * k1;
+        t[r] = (j == 0) ? u[4 * r + k1] : cmul(u[4 * r + k1], CO<C>::mk((RR)kC[j], s * (RR)kS[j]));
+      }
+      dft4<SIGN>(t[0], t[1], t[2], t[3]);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  } else if constexpr (R == 8) {
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<SIGN>(e0, e1, e2, e3);
+    dft4<SIGN>(o0, o1, o2, o3);
+    const RR h = (RR)0.70710678118654752440;
+    o1 = cmul(o1, CO<C>::mk(h, s * h));
+    o2 = cmul_iu(o2, s);
+    o3 = cmul(o3, CO<C>::mk(-h, s * h));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+  } else if constexpr (R <= 5) {
+    dft<SIGN>(R, v, tw, twn, twstride);
+  } else {  // primes 7, 11, 13: the plain DFT with table twiddles
+    C y[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      C acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int e = ((r * k) % R) * (twn / R);
+        acc = cadd(acc, cmul(v[r], twid<SIGN>(tw, e * twstride)));
+      }
+      y[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = y[k];
+  }
+}
+
+template <int SIGN, int R, typename C>
+__device__ inline void stockham_stage(const C* src, C* dst, int nl, int LS, int n, int Ns, const C* tw,
+                                      int twstride) {
+  const int nb = n / R;
+  const int total = nl * nb;
+  const int step = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int t = idx / nb, j = idx - t * nb;
+    const int jm = j % Ns;
+    C v[R];
+    const C* sp = src + t * LS + j;
+    v[0] = sp[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(sp[r * nb], twid<SIGN>(tw, (r * jm * step) * twstride));
+    dft_fixed<SIGN, R>(v, tw, n, twstride);
+    C* dp = dst + t * LS + (j / Ns) * Ns * R + jm;
+#pragma unroll
+    for (int k = 0; k < R; ++k) dp[k * Ns] = v[k];
+  }
+}
+
 // Stockham passes over `nl` lines of length n held in LDS (line stride LS).
 // Returns the buffer holding the natural-order result.
 template <int SIGN, typename C>
 __device__ C* stockham(C* A, C* B, int nl, int LS, const FFTStages& st, const C* tw, int twstride) {
   const int n = st.n;
-  const int twn = n;  // table index unit: W_n (times twstride in the table)
   C* src = A;
   C* dst = B;
   int Ns = 1;
   for (int s = 0; s < st.nst; ++s) {
     const int R = st.rad[s];
-    const int nb = n / R;
-    const int total = nl * nb;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      int t = idx / nb, j = idx - t * nb;
-      int jm = j % Ns;
-      C v[16];
-      const C* sp = src + t * LS + j;
-      v[0] = sp[0];
-      int step = n / (Ns * R);
-      for (int r = 1; r < R; ++r) v[r] = cmul(sp[r * nb], twid<SIGN>(tw, (r * jm * step) * twstride));
-      dft<SIGN>(R, v, tw, twn, twstride);
-      C* dp = dst + t * LS + (j / Ns) * Ns * R + jm;
-      for (int k = 0; k < R; ++k) dp[k * Ns] = v[k];
+    switch (R) {
+      case 2: stockham_stage<SIGN, 2>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 3: stockham_stage<SIGN, 3>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 4: stockham_stage<SIGN, 4>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 5: stockham_stage<SIGN, 5>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 8: stockham_stage<SIGN, 8>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 16: stockham_stage<SIGN, 16>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 7: stockham_stage<SIGN, 7>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      case 11: stockham_stage<SIGN, 11>(src, dst, nl, LS, n, Ns, tw, twstride); break;
+      default: stockham_stage<SIGN, 13>(src, dst, nl, LS, n, Ns, tw, twstride); break;  // make_stages: 13
     }
     __syncthreads();
     C* tmp = src;

@@ -537,11 +537,15 @@ __global__ void __launch_bounds__(128) k_pm1_stencil(PMLevel L, const XT<IZ>* __
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < nq) {
-      Acc s = 0;
-      for (int l = 0; l < 64; ++l) s += red[wave][lane][l];  // fixed order
-      ((Acc*)stencil)[(int64_t)(q0 + lane) * ncell + g] = s;
-    }
+    // 4 lanes per tap, each summing 16 lane partials, then the four parts
+    // combined through two shuffles: every sum in a fixed order
+    const int tq = lane % kRound, part = lane / kRound;
+    Acc s = 0;
+    if (part < 4 && tq < nq)
+      for (int l = 16 * part; l < 16 * part + 16; ++l) s += red[wave][tq][l];
+    s += __shfl(s, (lane + 2 * kRound) & 63, 64);
+    s += __shfl(s, (lane + kRound) & 63, 64);
+    if (lane < nq) ((Acc*)stencil)[(int64_t)(q0 + lane) * ncell + g] = s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

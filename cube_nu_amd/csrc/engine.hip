@@ -199,10 +199,11 @@ struct cube_p3m_ctx {
   bool classified = false;  // kick_pass: the classes are already on the host
   bool skip_zero = false;   // kick_pass: the reductions are already cleared
   bool no_trials = false;   // autotune: keep the first candidate, untimed
+  bool trial_skipped = false;  // set when no_trials kept a candidate untried
   int kph = 0;           // 0 both passes in one, 1 sizing only, 2 compute only
   bool graph_ok = true;  // cleared by a sizing pass that needs more than one batch per level
-  std::vector<uint64_t> gsig_acc, gsig;
-  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gsig_acc, gsig[2];
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // PM1+PM2, then PM3+PP(+buffer_v)
   int64_t graph_captures = 0, graph_launches = 0;
   DevBuf<double> kcoef;
   int64_t wsb = -1;  // workspace budget held for the duration of a kick
@@ -215,9 +216,11 @@ struct cube_p3m_ctx {
   DevBuf<char> ftune;  // the FFT shape trials' meshes (released after the trials)
   void sig_add(std::initializer_list<uint64_t> v) { gsig_acc.insert(gsig_acc.end(), v.begin(), v.end()); }
   void graph_drop() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-    gsig.clear();
+    for (int i = 0; i < 2; ++i) {
+      if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+      gexec[i] = nullptr;
+      gsig[i].clear();
+    }
   }
 
   ~cube_p3m_ctx() {
@@ -284,6 +287,11 @@ struct cube_p3m_ctx {
   template <typename R>
   void kick_overlapped(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v, std::vector<int32_t>& ires_h,
                        std::vector<int32_t>& cmax_h);
+  template <typename R>
+  bool classify_and_reserve(cube_kick_out& out, std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h);
+  template <typename R>
+  void kick_graph(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v, std::vector<int32_t>& ires_h,
+                  std::vector<int32_t>& cmax_h);
   template <typename R>
   void run_level(int level, int slot, const std::vector<PMBox>& bx, const float* G, const FFTCube<R>& fc,
                  const PMLevel& L, const cube_kick_in& in, PMOut* o, const char* tag, float* density_host = nullptr);
@@ -597,7 +605,10 @@ int cube_p3m_ctx::autotune(const std::string& key, const std::vector<int>& cand,
   // no trials while another stream computes (they would share its scratch and
   // see its velocities change): the first candidate now, the trials at the
   // next use on the first stream
-  if (no_trials) return cand[0];
+  if (no_trials) {
+    trial_skipped = true;
+    return cand[0];
+  }
   int pick = 0;
   struct Ev {  // destroyed on every exit, errors included
     hipEvent_t e = nullptr;
@@ -750,30 +761,13 @@ void cube_p3m_ctx::kick_pass(const cube_kick_in& in, cube_kick_out& out, float* 
   }
 }
 
-// Full kick with the PM3/PP sizing on the second stream, overlapped with the
-// PM1 and PM2 levels on the first (only the host-visible order changes: the
-// same kernels on the same data, so the result is the same).  The sizing
-// reads positions and counts only and writes the levels' own list buffers;
-// the workspace meshes are grown for every level before anything is queued,
-// so no buffer the first stream uses is reallocated while it runs.
+// Sub-tile classes (one read-back) and the workspace of the largest level,
+// grown before any level is queued; true when a PM3 or PP level needs several
+// workspace batches (its lists then cannot be built ahead of its compute).
 template <typename R>
-void cube_p3m_ctx::kick_overlapped(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v,
-                                   std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h) {
+bool cube_p3m_ctx::classify_and_reserve(cube_kick_out& out, std::vector<int32_t>& ires_h,
+                                        std::vector<int32_t>& cmax_h) {
   using C = typename Cplx<R>::T;
-  struct Reset {  // one-pass kicks on the first stream again on every exit, errors included
-    cube_p3m_ctx* c;
-    hipStream_t s0;
-    bool timing;
-    ~Reset() {
-      c->s = s0;
-      c->kph = 0;
-      c->classified = false;
-      c->skip_zero = false;
-      c->no_trials = false;
-      c->tm.enabled = timing;
-    }
-  } reset{this, s, tm.enabled};
-  // 1. classes (one read-back) and the largest level workspace
   classify_subtiles(g, cfg.ires_fixed, rhocc(), ires.p, napm.p, cmax.p, s);
   CUBE_HIP(hipMemcpyAsync(ires_h.data(), ires.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
   CUBE_HIP(hipMemcpyAsync(cmax_h.data(), cmax.p, sizeof(int32_t) * g.ns3, hipMemcpyDeviceToHost, s));
@@ -801,6 +795,33 @@ void cube_p3m_ctx::kick_overlapped(const cube_kick_in& in, cube_kick_out& out, b
     }
     work.ensure((size_t)need);
   }
+  return multi;
+}
+
+// Full kick with the PM3/PP sizing on the second stream, overlapped with the
+// PM1 and PM2 levels on the first (only the host-visible order changes: the
+// same kernels on the same data, so the result is the same).  The sizing
+// reads positions and counts only and writes the levels' own list buffers;
+// the workspace meshes are grown for every level before anything is queued,
+// so no buffer the first stream uses is reallocated while it runs.
+template <typename R>
+void cube_p3m_ctx::kick_overlapped(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v,
+                                   std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h) {
+  struct Reset {  // one-pass kicks on the first stream again on every exit, errors included
+    cube_p3m_ctx* c;
+    hipStream_t s0;
+    bool timing;
+    ~Reset() {
+      c->s = s0;
+      c->kph = 0;
+      c->classified = false;
+      c->skip_zero = false;
+      c->no_trials = false;
+      c->tm.enabled = timing;
+    }
+  } reset{this, s, tm.enabled};
+  // 1. classes (one read-back) and the largest level workspace
+  const bool multi = classify_and_reserve<R>(out, ires_h, cmax_h);
   if (multi) {  // lists per batch: PM1/PM2, then PM3/PP, in one pass each on the first stream
     kph = 0;
     kick_pass<R>(in, out, nullptr, 3, false, ires_h, cmax_h);
@@ -833,6 +854,107 @@ void cube_p3m_ctx::kick_overlapped(const cube_kick_in& in, cube_kick_out& out, b
   kick_pass<R>(in, out, nullptr, 12, with_buffer_v, ires_h, cmax_h);
 }
 
+// Graph mode: the overlapped kick's two compute parts as two graphs.  Part 0
+// (PM1 + PM2) is sized and launched on the library stream; part 1 (PM3 + PP +
+// buffer_v) is sized on the second stream meanwhile and launched after it.
+// Each part is captured on its first use and replayed while its launch
+// signature holds.  A part whose sizing left autotuner trials untried runs
+// uncaptured (the trials run there, on the library stream) and is captured at
+// a later kick.
+template <typename R>
+void cube_p3m_ctx::kick_graph(const cube_kick_in& in, cube_kick_out& out, bool with_buffer_v,
+                              std::vector<int32_t>& ires_h, std::vector<int32_t>& cmax_h) {
+  struct Reset {
+    cube_p3m_ctx* c;
+    hipStream_t s0;
+    bool timing;
+    ~Reset() {
+      c->s = s0;
+      c->kph = 0;
+      c->classified = false;
+      c->skip_zero = false;
+      c->no_trials = false;
+      c->tm.enabled = timing;
+    }
+  } reset{this, s, tm.enabled};
+  const hipStream_t s0 = s;
+  if (classify_and_reserve<R>(out, ires_h, cmax_h)) {
+    // several workspace batches in a level: uncaptured, one pass per part
+    graph_drop();
+    kph = 0;
+    kick_pass<R>(in, out, nullptr, 3, false, ires_h, cmax_h);
+    skip_zero = true;
+    kick_pass<R>(in, out, nullptr, 12, with_buffer_v, ires_h, cmax_h);
+    return;
+  }
+  kcoef.ensure(1);
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, s, kcoef.p, kick_coef(in));
+  CUBE_HIP(hipGetLastError());
+  auto common = [&](int part) {
+    sig_add({(uint64_t)part, (uint64_t)prec, fbits(in.mass_p_cdm), (uint64_t)(part == 1 && with_buffer_v),
+             (uint64_t)(uintptr_t)xpc(), (uint64_t)(uintptr_t)vpc(), (uint64_t)(uintptr_t)rhocc(),
+             (uint64_t)(uintptr_t)off.p, (uint64_t)(uintptr_t)pmout.p, (uint64_t)(uintptr_t)ppout.p,
+             (uint64_t)(uintptr_t)kcoef.p});
+  };
+  // capture (when the signature changed) and launch one part's compute pass
+  auto launch = [&](int part, int pmask, bool bufv) {
+    kph = 2;
+    if (!gexec[part] || gsig_acc != gsig[part]) {
+      if (gexec[part]) (void)hipGraphExecDestroy(gexec[part]);
+      gexec[part] = nullptr;
+      gsig[part].clear();
+      hipGraph_t gr = nullptr;
+      CUBE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      try {
+        kick_pass<R>(in, out, nullptr, pmask, bufv, ires_h, cmax_h);
+      } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(s, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+      }
+      CUBE_HIP(hipStreamEndCapture(s, &gr));
+      const hipError_t e = hipGraphInstantiate(&gexec[part], gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        gexec[part] = nullptr;
+        throw Error(CUBE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      gsig[part] = gsig_acc;
+      ++graph_captures;
+    }
+    CUBE_HIP(hipGraphLaunch(gexec[part], s));
+  };
+  tm.enabled = false;
+  // part 0: PM1 + PM2 (its sizing is box tables, stencil and, at first use,
+  // the PM2 FFT shape trials)
+  kph = 1;
+  gsig_acc.clear();
+  kick_pass<R>(in, out, nullptr, 3, false, ires_h, cmax_h);
+  common(0);
+  launch(0, 3, false);
+  // part 1 sizing on the second stream, without trials
+  s = s2;
+  kph = 1;
+  gsig_acc.clear();
+  no_trials = true;
+  trial_skipped = false;
+  kick_pass<R>(in, out, nullptr, 12, false, ires_h, cmax_h);
+  no_trials = false;
+  CUBE_HIP(hipEventRecord(ev_sized, s2));
+  s = s0;
+  CUBE_HIP(hipStreamWaitEvent(s, ev_sized, 0));
+  skip_zero = true;
+  if (trial_skipped) {
+    kph = 2;  // uncaptured: the trials run now, on the library stream
+    kick_pass<R>(in, out, nullptr, 12, with_buffer_v, ires_h, cmax_h);
+  } else {
+    common(1);
+    launch(1, 12, with_buffer_v);
+    ++graph_launches;
+  }
+}
+
 template <typename R>
 void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* density_host, bool with_buffer_v) {
   if (layout != 1) throw Error(CUBE_ERR_STATE, "kick needs the buffered (extended) particle layout");
@@ -856,70 +978,7 @@ void cube_p3m_ctx::kick(const cube_kick_in& in, cube_kick_out& out, float* densi
     kph = 0;
     kick_pass<R>(in, out, density_host, mask, with_buffer_v, ires_h, cmax_h);
   } else {
-    struct Phase {  // back to one-pass kicks on every exit, errors included
-      int& k;
-      ~Phase() { k = 0; }
-    } ph{kph};
-    const bool timing = tm.enabled;
-    {
-      KScope k(&tm, "kick_sizing", 0);
-      tm.enabled = false;
-      kph = 1;
-      graph_ok = true;
-      gsig_acc.clear();
-      try {
-        kick_pass<R>(in, out, nullptr, mask, with_buffer_v, ires_h, cmax_h);
-      } catch (...) {
-        tm.enabled = timing;
-        throw;
-      }
-      tm.enabled = timing;
-    }
-    kcoef.ensure(1);
-    hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, s, kcoef.p, kick_coef(in));
-    CUBE_HIP(hipGetLastError());
-    if (!graph_ok) {
-      // a level needs several workspace batches, whose lists share one set of
-      // buffers: the kick runs again in one pass, uncaptured (the sizing pass
-      // wrote lists and box tables only)
-      graph_drop();
-      kph = 0;
-      kick_pass<R>(in, out, nullptr, mask, with_buffer_v, ires_h, cmax_h);
-    } else {
-      kph = 2;
-      sig_add({(uint64_t)mask, (uint64_t)prec, fbits(in.mass_p_cdm), (uint64_t)with_buffer_v,
-               (uint64_t)(uintptr_t)xpc(), (uint64_t)(uintptr_t)vpc(), (uint64_t)(uintptr_t)rhocc(),
-               (uint64_t)(uintptr_t)off.p, (uint64_t)(uintptr_t)pmout.p, (uint64_t)(uintptr_t)ppout.p,
-               (uint64_t)(uintptr_t)kcoef.p});
-      KScope k(&tm, "kick_graph", 0);
-      if (!gexec || gsig_acc != gsig) {
-        graph_drop();
-        tm.enabled = false;
-        hipGraph_t gr = nullptr;
-        CUBE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-        try {
-          kick_pass<R>(in, out, nullptr, mask, with_buffer_v, ires_h, cmax_h);
-        } catch (...) {
-          tm.enabled = timing;
-          hipGraph_t tmp = nullptr;
-          (void)hipStreamEndCapture(s, &tmp);
-          if (tmp) (void)hipGraphDestroy(tmp);
-          throw;
-        }
-        tm.enabled = timing;
-        CUBE_HIP(hipStreamEndCapture(s, &gr));
-        const hipError_t e = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e != hipSuccess) {
-          gexec = nullptr;
-          throw Error(CUBE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        }
-        gsig = gsig_acc;
-        ++graph_captures;
-      }
-      CUBE_HIP(hipGraphLaunch(gexec, s));
-      ++graph_launches;
-    }
+    kick_graph<R>(in, out, with_buffer_v, ires_h, cmax_h);
   }
   // collect reductions
   PMOut po[4];

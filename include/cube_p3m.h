@@ -240,15 +240,15 @@ int cube_p3m_kick_resident_nu(cube_p3m_ctx* ctx, cube_p3m_ctx* nu,
 int cube_p3m_add_thermal_velocities(cube_p3m_ctx* ctx, double v_scale,
                                     uint64_t seed, float* vz_max);
 /* Graph mode for full resident kicks (and the kick + buffer_v of
- * cube_p3m_step_resident): the kick's sizing pass (sub-tile classes, list
- * counts and fills) runs as usual and its compute pass (deposits, FFTs,
- * interpolation + kick, PP force, buffer_v) is captured into a HIP graph
- * once and replayed while the launch signature (box counts per level and
- * the buffer addresses) is unchanged; the kick coefficient is read from
- * device memory.  Results are bit-identical to graph mode off.  Levels that
- * need more than one workspace batch run uncaptured.  Counts of captures and
- * launches appear in the timing report (graph_captures, graph_launches).
- * No reference counterpart (the reference's step loop is main.f90:30-59). */
+ * cube_p3m_step_resident): the kick's compute passes -- PM1 + PM2, then
+ * PM3 + PP (+ buffer_v), whose lists are built on a second stream meanwhile
+ * -- are captured into two HIP graphs and replayed while their launch
+ * signatures (box counts per level, tuned shapes, buffer addresses) are
+ * unchanged; the kick coefficient is read from device memory.  Results are
+ * bit-identical to graph mode off.  Kicks with a level that needs more than
+ * one workspace batch run uncaptured.  The timing report counts captures and
+ * graph-run kicks (graph_captures, graph_launches).  No reference
+ * counterpart (the reference's step loop is main.f90:30-59). */
 int cube_p3m_graph_enable(cube_p3m_ctx* ctx, int32_t enable);
 int cube_p3m_timing_enable(cube_p3m_ctx* ctx, int32_t enable);
 int cube_p3m_timing_reset(cube_p3m_ctx* ctx);

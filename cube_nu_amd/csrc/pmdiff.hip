@@ -330,6 +330,117 @@ __global__ void k_tsc_deposit_vec(const float* __restrict__ x, const float* __re
   }
 }
 
+// Tiled deposits.  The mesh is cut into kTile^3-cell tiles; a particle is
+// listed under every tile its 3^3 TSC taps reach (one to eight), and one
+// workgroup per tile sums the taps that fall inside it in LDS and writes the
+// tile's cells once: one global atomic per listing instead of 27 (or 81) per
+// particle.  The contributions are the integers the one-thread-per-particle
+// form adds, so every sum, and every output, is the same bit for bit.
+constexpr int kTile = 8;
+
+__device__ inline void tile_span(int i, int n, int& t0, int& t1) {
+  // tiles of the taps i-1 .. i+1 (periodic); t1 == t0 when they share one
+  const int a = wrapi(i - 1, n) / kTile, b = wrapi(i + 1, n) / kTile;
+  t0 = a;
+  t1 = b;
+}
+
+template <int FILL>
+__global__ void k_tile_bin(const float* __restrict__ x, int64_t np, int64_t ntot, int n, Lattice lt,
+                           int* __restrict__ count, const int64_t* __restrict__ start, int* __restrict__ cursor,
+                           int* __restrict__ ents) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int64_t b = p / np;
+  const int nt = n / kTile;
+  int t0[3], t1[3];
+  for (int d = 0; d < 3; ++d) {
+    int i;
+    float w[3], dw[3];
+    tsc_pos(x, p, d, lt, i, w, dw);
+    tile_span(i, n, t0[d], t1[d]);
+  }
+  for (int cz = 0; cz < 2; ++cz) {
+    if (cz && t1[2] == t0[2]) break;
+    const int tz = cz ? t1[2] : t0[2];
+    for (int cy = 0; cy < 2; ++cy) {
+      if (cy && t1[1] == t0[1]) break;
+      const int ty = cy ? t1[1] : t0[1];
+      for (int cx = 0; cx < 2; ++cx) {
+        if (cx && t1[0] == t0[0]) break;
+        const int tx = cx ? t1[0] : t0[0];
+        const int64_t t = ((b * nt + tz) * nt + ty) * nt + tx;
+        if (FILL) ents[start[t] + atomicAdd(&cursor[t], 1)] = (int)p;
+        else atomicAdd(&count[t], 1);
+      }
+    }
+  }
+}
+
+// One workgroup per tile: the listed particles' taps inside the tile, summed
+// in LDS (u64; signed values as two's complement), then the tile's cells
+// written as floats: NC = 1, the density (contribution u32(W 2^32), out =
+// sum / 2^32); NC = 3, the adjoint's vector (contribution int(W g S), out =
+// sum / S).
+template <int NC>
+__global__ void __launch_bounds__(256) k_tile_deposit(const float* __restrict__ x, const float* __restrict__ g,
+                                                      int64_t np, int n, Lattice lt,
+                                                      const int64_t* __restrict__ start,
+                                                      const int* __restrict__ ents,
+                                                      const unsigned int* __restrict__ gmax, float* __restrict__ out) {
+  __shared__ unsigned long long acc[NC][kTile * kTile * kTile];
+  const int nt = n / kTile;
+  const int64_t t = blockIdx.x;
+  const int tx = (int)(t % nt), ty = (int)((t / nt) % nt), tz = (int)((t / ((int64_t)nt * nt)) % nt);
+  const int64_t b = t / ((int64_t)nt * nt * nt);
+  for (int q = threadIdx.x; q < NC * kTile * kTile * kTile; q += blockDim.x) (&acc[0][0])[q] = 0;
+  __syncthreads();
+  const double S = NC == 3 ? fix_scale(gmax) : 1.0;
+  const int org[3] = {tx * kTile, ty * kTile, tz * kTile};
+  for (int64_t e = start[t] + threadIdx.x; e < start[t + 1]; e += blockDim.x) {
+    const int64_t p = ents[e];
+    int i[3];
+    float w[3][3], dw[3][3];
+    for (int d = 0; d < 3; ++d) tsc_pos(x, p, d, lt, i[d], w[d], dw[d]);
+    float gs[3] = {0, 0, 0};
+    if (NC == 3)
+      for (int d = 0; d < 3; ++d) gs[d] = (float)(g[3 * p + d] * S);
+    for (int k3 = 0; k3 < 3; ++k3) {
+      const int lz = wrapi(i[2] - 1 + k3, n) - org[2];
+      if (lz < 0 || lz >= kTile) continue;
+      for (int k2 = 0; k2 < 3; ++k2) {
+        const int ly = wrapi(i[1] - 1 + k2, n) - org[1];
+        if (ly < 0 || ly >= kTile) continue;
+        for (int k1 = 0; k1 < 3; ++k1) {
+          const int lx = wrapi(i[0] - 1 + k1, n) - org[0];
+          if (lx < 0 || lx >= kTile) continue;
+          const int m = (lz * kTile + ly) * kTile + lx;
+          if (NC == 1) {
+            const float wyz32 = (w[1][k2] * w[2][k3]) * 4294967296.0f;
+            atomicAdd(&acc[0][m], (unsigned long long)__float2uint_rn(w[0][k1] * wyz32));
+          } else {
+            const float wt = w[0][k1] * (w[1][k2] * w[2][k3]);
+            for (int d = 0; d < NC; ++d) {
+              const int c = __float2int_rn(wt * gs[d]);
+              if (c != 0) atomicAdd(&acc[d][m], (unsigned long long)(long long)c);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t n3 = (int64_t)n * n * n;
+  for (int q = threadIdx.x; q < kTile * kTile * kTile; q += blockDim.x) {
+    const int lx = q % kTile, ly = (q / kTile) % kTile, lz = q / (kTile * kTile);
+    const int64_t m = ((int64_t)(org[2] + lz) * n + (org[1] + ly)) * n + (org[0] + lx);
+    for (int d = 0; d < NC; ++d) {
+      const unsigned long long v = acc[d][q];
+      out[(b * NC + d) * n3 + m] = NC == 1 ? (float)((double)v * (1.0 / kFix)) : (float)((double)(long long)v / S);
+    }
+  }
+}
+
 // out = (signed) acc / S, acc cleared.
 __global__ void k_sfix_to_float(unsigned long long* __restrict__ acc, int64_t n, const unsigned int* __restrict__ gmax,
                                 float* __restrict__ out) {
@@ -429,9 +540,37 @@ struct cube_pm_ctx {
   PMBuf<unsigned long long> acc;   // B*3*n^3, kept zero between calls
   PMBuf<unsigned long long> accw;  // B*nw^3 for the CIC density, kept zero
   PMBuf<unsigned int> gmax;
+  // tiled deposits: per-tile counts, list starts, fill cursors, particle lists
+  PMBuf<int> tcount, tcursor, tents;
+  PMBuf<int64_t> tstart;
+  PMBuf<char> tscan;
   int nw_alloc = 0;
   Lattice lat;
   std::string err;
+
+  bool tiled() const { return cfg.n % kTile == 0; }
+  int64_t ntiles() const {
+    const int64_t t = cfg.n / kTile;
+    return (int64_t)cfg.batch * t * t * t;
+  }
+  // list every particle of x under the tiles its TSC taps reach
+  void bin(const float* x, int64_t np, hipStream_t s) {
+    const int n = cfg.n;
+    const int64_t ntot = np * cfg.batch, nt = ntiles();
+    tcount.ensure((size_t)nt);
+    tcursor.ensure((size_t)nt);
+    tstart.ensure((size_t)nt + 1);
+    tents.ensure((size_t)ntot * 8);  // at most eight tiles per particle
+    tscan.ensure(scan_tmp_bytes(nt));
+    hipLaunchKernelGGL(k_zero_u32, dim3(nblk(nt)), dim3(256), 0, s, (unsigned int*)tcount.p, nt);
+    hipLaunchKernelGGL(k_zero_u32, dim3(nblk(nt)), dim3(256), 0, s, (unsigned int*)tcursor.p, nt);
+    hipLaunchKernelGGL((k_tile_bin<0>), dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, lat, tcount.p, nullptr,
+                       nullptr, nullptr);
+    scan_counts(tcount.p, tstart.p, nt, tscan.p, tscan.n, s);
+    hipLaunchKernelGGL((k_tile_bin<1>), dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, lat, nullptr, tstart.p,
+                       tcursor.p, tents.p);
+    CUBE_HIP(hipGetLastError());
+  }
 
   void check_np(int64_t np) const {
     if (np <= 0) throw Error(CUBE_ERR_ARG, "np must be positive");
@@ -442,8 +581,14 @@ struct cube_pm_ctx {
   void force_meshes(const float* x, int64_t np, float* F, hipStream_t s) {
     const int n = cfg.n, B = cfg.batch;
     const int64_t ntot = np * B;
-    hipLaunchKernelGGL(k_tsc_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, lat, acc.p);
-    hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * n3)), dim3(256), 0, s, acc.p, B * n3, 1.0 / kFix, mesh.p);
+    if (tiled()) {
+      bin(x, np, s);
+      hipLaunchKernelGGL((k_tile_deposit<1>), dim3((unsigned)ntiles()), dim3(256), 0, s, x, nullptr, np, n, lat,
+                         tstart.p, tents.p, nullptr, mesh.p);
+    } else {
+      hipLaunchKernelGGL(k_tsc_deposit, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, lat, acc.p);
+      hipLaunchKernelGGL(k_fix_to_float, dim3(nblk(B * n3)), dim3(256), 0, s, acc.p, B * n3, 1.0 / kFix, mesh.p);
+    }
     fft_convolve<float>(fc, mesh.p, work.p, G.p, B, (float)n3, 0.0f, s);
     hipLaunchKernelGGL(k_fd_force, dim3(nblk(B * n3)), dim3(256), 0, s, mesh.p, B, n, F);
     CUBE_HIP(hipGetLastError());
@@ -549,17 +694,24 @@ int cube_pm_force_vjp(cube_pm_ctx* ctx, const float* x, int64_t np, const float*
     const hipStream_t s = (hipStream_t)stream;
     const int n = ctx->cfg.n, B = ctx->cfg.batch;
     const int64_t ntot = np * B, n3 = ctx->n3;
-    if (!F) {  // recompute the force meshes of x (checkpoint-and-recompute callers)
+    if (!F) {  // recompute the force meshes of x (checkpoint-and-recompute callers); bins x when tiled
       ctx->force_meshes(x, np, ctx->Fs.p, s);
       F = ctx->Fs.p;
+    } else if (ctx->tiled()) {
+      ctx->bin(x, np, s);
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctx->gmax.p, 1);
     hipLaunchKernelGGL(k_absmax, dim3(std::min<int64_t>(nblk(3 * ntot), kMaxBlocks)), dim3(256), 0, s, g_acc,
                        3 * ntot, ctx->gmax.p);
-    hipLaunchKernelGGL(k_tsc_deposit_vec, dim3(nblk(ntot)), dim3(256), 0, s, x, g_acc, np, ntot, n, ctx->lat,
-                       ctx->gmax.p, ctx->acc.p);
-    hipLaunchKernelGGL(k_sfix_to_float, dim3(nblk(3 * B * n3)), dim3(256), 0, s, ctx->acc.p, 3 * B * n3, ctx->gmax.p,
-                       ctx->Fbar.p);
+    if (ctx->tiled()) {
+      hipLaunchKernelGGL((k_tile_deposit<3>), dim3((unsigned)ctx->ntiles()), dim3(256), 0, s, x, g_acc, np, n,
+                         ctx->lat, ctx->tstart.p, ctx->tents.p, ctx->gmax.p, ctx->Fbar.p);
+    } else {
+      hipLaunchKernelGGL(k_tsc_deposit_vec, dim3(nblk(ntot)), dim3(256), 0, s, x, g_acc, np, ntot, n, ctx->lat,
+                         ctx->gmax.p, ctx->acc.p);
+      hipLaunchKernelGGL(k_sfix_to_float, dim3(nblk(3 * B * n3)), dim3(256), 0, s, ctx->acc.p, 3 * B * n3,
+                         ctx->gmax.p, ctx->Fbar.p);
+    }
     hipLaunchKernelGGL(k_fd_adjoint, dim3(nblk(B * n3)), dim3(256), 0, s, ctx->Fbar.p, B, n, ctx->mesh.p);
     fft_convolve<float>(ctx->fc, ctx->mesh.p, ctx->work.p, ctx->G.p, B, (float)n3, 0.0f, s);
     hipLaunchKernelGGL(k_tsc_vjp_gather, dim3(nblk(ntot)), dim3(256), 0, s, x, np, ntot, n, ctx->lat, F, g_acc,

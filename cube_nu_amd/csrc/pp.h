@@ -60,6 +60,6 @@ void pp_force(const PPLevel& L, int precision, const PPBox* boxes_dev, int nbox,
               hipStream_t s, int unroll = 6);
 
 // Entries past the last one that the force kernel may read (and ignore).
-constexpr int kPPPad = 8;
+constexpr int kPPPad = 16;
 
 }  // namespace cube

@@ -125,10 +125,15 @@ __global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restri
 }
 
 constexpr int PB = 4;       // physical PP cells per brick per dim
+// Lanes per target in the FP32 force kernel: a target's candidates are dealt
+// to kPPGroup lanes in turn and their partial sums combined at the end (26.1
+// ms at 512^3 against 27.3 with one lane per target, 27.2 with four, 31.5
+// with eight: pairs of lanes read consecutive entries and wait less on the
+// longest column of the wavefront).
+constexpr int kPPGroup = 2;
 // Candidate loads in flight per lane: U in {4, 6, 8}, chosen per PP level at
 // run time by the engine's autotuner (every U sums the pairs in the same
 // order, so the choice never changes a result).
-static_assert(8 <= kPPPad, "entries are padded for the largest U of reads past a run");
 
 
 // F(r, r_pp) = c Q(ep), ep = 2r/r_pp < 2, with Q(ep) = pi(ep) for ep <= 1 and
@@ -234,8 +239,11 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // densest sub-tile first (kick.f90:132-138 sorts by npc_max for
 // schedule(dynamic,1), kick.f90:285), so every counter drains the heaviest
 // bricks first and the launch does not end on a few heavy workgroups.  In a
-// brick each thread owns target particles and sums the pair forces of the
-// 27 neighbouring PP cells in the fixed (dz, dy, x) order.  PP cells are split into kSX sub-bins along x and
+// brick each group of G lanes owns a target particle and sums the pair forces
+// of the 27 neighbouring PP cells in the fixed (dz, dy, x) order, lane j of
+// the group taking candidates j, j + G, ... of each run; the G partial sums
+// are added by a fixed butterfly, so the result is deterministic.  PP cells
+// are split into kSX sub-bins along x and
 // the CSR is x-ordered along each (y, z) row, so the part of a (dy, dz)
 // column that can hold partners, x in [x_t - r_pp, x_t + r_pp], is one
 // contiguous run found from two sub-bin starts; columns whose y/z distance
@@ -244,14 +252,21 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // here with the reference's FP32 operations and the interacting set is the
 // reference's.  The sub-bin starts of the (PB+2)^3 neighbourhood are staged
 // in LDS; candidates are read U at a time from L1/L2 (the entry array is
-// padded by kPPPad >= U entries so the reads need no clamp).  A two-pass form
+// padded by kPPPad > (U-1) G entries so the reads need no clamp).
+// Measured at 512^3 (texture address and data units ~90-100% busy, VALU ~65%
+// busy in the chosen form), slower variants: a two-pass form
 // (in-range tests of 32 candidates into a bit mask, then the pair law over
 // the set bits only, each partner read again) measured 37.9 against 27.2 ms
 // at 512^3: the second pass's reads come from L2, one per pair.  Staging
 // each brick's neighbourhood (up to 2560 entries) in LDS and reading the
 // candidates from there measured 50.3 ms: 50 KB of LDS per workgroup leaves
 // three wavefronts per SIMD, and the lanes' scattered 16-byte LDS reads
-// conflict.
+// conflict.  8-byte entries (positions as exact integers in units of 2^-15
+// fine cells, differences formed exactly through 2^23 + n in FP32) halved the
+// load instructions but added ~30% VALU work: 30.3 ms.  Deferring the pair
+// law until some lane holds two pairs (so it runs on most lanes at once)
+// cut the VALU count but its branches and spills gave 32.4 ms.  Splitting PP
+// cells also along y (fewer candidates, twice the columns) gave 30.2 ms.
 // FP32 form: eight 256-thread workgroups per CU (the launch bound's second
 // value), i.e. at most 64 VGPRs and eight wavefronts per SIMD to cover the
 // candidate loads' latency (28.5 vs 30.2 ms at 512^3 with 71 VGPRs and seven
@@ -262,7 +277,7 @@ constexpr int pp_min_blocks_per_cu() {
 }
 constexpr int kPPClaimCounters = 8;
 
-template <typename R, int U>
+template <typename R, int U, int G>
 __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   int64_t nbricks, const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
@@ -328,7 +343,8 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
     const PPBox B = boxes[ib];
     const float xoff[3] = {(float)(B.ifs[0] * L.ntt), (float)(B.ifs[1] * L.ntt), (float)(B.ifs[2] * L.ntt)};
     const int ntg = pre[PB * PB * PB];
-    for (int t = threadIdx.x; t < ntg; t += blockDim.x) {
+    const int gj = (int)threadIdx.x % G;  // lane within the target's group
+    for (int t = (int)threadIdx.x / G; t < ntg; t += (int)blockDim.x / G) {
       int lo = 0, hi = PB * PB * PB - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -371,34 +387,40 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
           if (gyz) ph0 = ph1 = 0;
           const unsigned phn = (unsigned)(ph1 - ph0);
           const float4* run = ent + beg;
-          for (int p = 0; p < nn; p += U) {
+          for (int p = gj; p < nn; p += U * G) {
             float4 o[U];
   #pragma unroll
-            for (int k = 0; k < U; ++k) o[k] = run[p + k];
+            for (int k = 0; k < U; ++k) o[k] = run[p + k * G];
   #pragma unroll
             for (int k = 0; k < U; ++k) {
               const float rv0 = o[k].x - me.x, rv1 = o[k].y - me.y, rv2 = o[k].z - me.z;
               const float r2 = (rv0 * rv0 + rv1 * rv1) + rv2 * rv2;  // exact-input FP32 test (kick.f90:525, 556)
-              if (p + k < nn && 0.0f < r2 && r2 < L.pp_range2) {
+              if (p + k * G < nn && 0.0f < r2 && r2 < L.pp_range2) {
                 pair_force<R>(L, qtab, me, o[k], f1);
                 ++npair;
-                nghost += (unsigned)(p + k - ph0) >= phn;
+                nghost += (unsigned)(p + k * G - ph0) >= phn;
               }
             }
           }
         }
       npg += (unsigned)nghost;
       npp += (unsigned)(npair - nghost);
-      // update v (kick.f90:576-582): g = fl32(f mass_scale) as the reference
-      // forms it, times the FP64 coefficient, added in FP64
-      const unsigned ip = __float_as_uint(me.w);
-      float g[3];
-      for (int d = 0; d < 3; ++d) g[d] = (float)f1[d] * L.mass_scale;
-      f2 = fmaxf(f2, (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
-      for (int d = 0; d < 3; ++d) {
-        const float v = (float)((double)vp[3 * (int64_t)ip + d] + (double)g[d] * vsc);
-        vp[3 * (int64_t)ip + d] = v;
-        vm[d] = fmaxf(vm[d], fabsf(v));
+      // the group's partial sums, combined by a butterfly: every lane ends
+      // with the same value (each step adds the same two operands)
+      for (int o = G / 2; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) f1[d] += __shfl_xor(f1[d], o, 64);
+      if (gj == 0) {
+        // update v (kick.f90:576-582): g = fl32(f mass_scale) as the
+        // reference forms it, times the FP64 coefficient, added in FP64
+        const unsigned ip = __float_as_uint(me.w);
+        float g[3];
+        for (int d = 0; d < 3; ++d) g[d] = (float)f1[d] * L.mass_scale;
+        f2 = fmaxf(f2, (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+        for (int d = 0; d < 3; ++d) {
+          const float v = (float)((double)vp[3 * (int64_t)ip + d] + (double)g[d] * vsc);
+          vp[3 * (int64_t)ip + d] = v;
+          vm[d] = fmaxf(vm[d], fabsf(v));
+        }
       }
     }
   }  // claimed bricks
@@ -470,6 +492,24 @@ void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, cons
   CUBE_HIP(hipGetLastError());
 }
 
+template <int U, int G>
+static void launch_pp_f32(dim3 grid, hipStream_t s, const PPLevel& L, const PPBox* boxes, int64_t nbricks,
+                          PPScratch& S, float* vp, PPOut* out) {
+  static_assert((U - 1) * G < kPPPad, "reads past a run stay inside the entry padding");
+  hipLaunchKernelGGL((k_pp_force<float, U, G>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
+                     out, S.claim);
+}
+
+template <int G>
+static void launch_pp_f32_u(int unroll, dim3 grid, hipStream_t s, const PPLevel& L, const PPBox* boxes,
+                            int64_t nbricks, PPScratch& S, float* vp, PPOut* out) {
+  switch (unroll) {
+    case 4: launch_pp_f32<4, G>(grid, s, L, boxes, nbricks, S, vp, out); break;
+    case 8: launch_pp_f32<8, G>(grid, s, L, boxes, nbricks, S, vp, out); break;
+    default: launch_pp_f32<6, G>(grid, s, L, boxes, nbricks, S, vp, out); break;
+  }
+}
+
 void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, float* vp, PPScratch& S, PPOut* out,
               hipStream_t s, int unroll) {
   const int64_t nbr = (L.npgrid + PB - 1) / PB;
@@ -488,17 +528,10 @@ void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, flo
   const int per_cu = precision == 64 ? 4 : pp_min_blocks_per_cu<float>();
   const dim3 grid((unsigned)std::min<int64_t>(nbricks, (int64_t)n_cu * per_cu));
   if (precision == 64)
-    hipLaunchKernelGGL((k_pp_force<double, 6>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
-                       out, S.claim);
-  else if (unroll == 4)
-    hipLaunchKernelGGL((k_pp_force<float, 4>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
-                       out, S.claim);
-  else if (unroll == 8)
-    hipLaunchKernelGGL((k_pp_force<float, 8>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
-                       out, S.claim);
+    hipLaunchKernelGGL((k_pp_force<double, 6, 1>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries,
+                       vp, out, S.claim);
   else
-    hipLaunchKernelGGL((k_pp_force<float, 6>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
-                       out, S.claim);
+    launch_pp_f32_u<kPPGroup>(unroll, grid, s, L, boxes, nbricks, S, vp, out);
   CUBE_HIP(hipGetLastError());
 }
 

@@ -235,7 +235,9 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 
 // Persistent: each workgroup claims bricks of PB^3 physical PP cells from one
 // of kPPClaimCounters counters (workgroup i uses counter i % 8, which hands
-// out the bricks 8c + i % 8) until they run out.  The host orders the boxes
+// out the bricks of sub-tiles i % 8, i % 8 + 8, ..., one sub-tile at a time,
+// so that a sub-tile's bricks run on one XCD and share its L2) until they run
+// out.  The host orders the boxes
 // densest sub-tile first (kick.f90:132-138 sorts by npc_max for
 // schedule(dynamic,1), kick.f90:285), so every counter drains the heaviest
 // bricks first and the launch does not end on a few heavy workgroups.  In a
@@ -301,7 +303,15 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
   unsigned long long npp = 0, npg = 0;
   for (;;) {
     __syncthreads();  // the previous brick's tables are no longer read
-    if (threadIdx.x == 0) s_brick = (int64_t)atomicAdd(&claim[ctr], 1u) * kPPClaimCounters + ctr;
+    if (threadIdx.x == 0) {
+      // counter ctr hands out the bricks of sub-tiles ctr, ctr + 8, ... (in
+      // the host's densest-first order), one sub-tile's bricks in turn: its
+      // workgroups (blockIdx % 8 == ctr) share an XCD, whose L2 then holds
+      // the neighbourhoods that a sub-tile's bricks read in common
+      const int64_t k = (int64_t)atomicAdd(&claim[ctr], 1u), nbps = (int64_t)nbr * nbr * nbr;
+      const int64_t box = (k / nbps) * kPPClaimCounters + ctr;
+      s_brick = box < nbricks / nbps ? box * nbps + k % nbps : nbricks;
+    }
     __syncthreads();
     int64_t b = s_brick;
     if (b >= nbricks) break;

@@ -298,6 +298,10 @@ struct cube_p3m_ctx {
   void pp_phase(const std::vector<int32_t>& ires_h, const std::vector<int32_t>& cmax_h, const cube_kick_in& in,
                 PPOut* o);
   int tune_pp(int iapm, const PPLevel& L, const PPBox* boxes, int nb, PPScratch& S);
+  template <typename R>
+  int interp_shape(int level, int n, const PMLevel& L, const PMBox* boxes, int nb, const cube_kick_in& in, const R* phi,
+                   bool trials);
+  DevBuf<PMOut> pmtune;
   std::string tune_key(const char* what, int level, int64_t size) const {
     return std::string(what) + "_l" + std::to_string(level) + "_n" + std::to_string(size) + "_fp" +
            std::to_string(prec);
@@ -453,6 +457,8 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
       if (trials) ftune.release();
       if (SZ) sig_add({(uint64_t)fshape});
     }
+    // the interpolation's block shape so far (its trials need the potential)
+    if (SZ) sig_add({(uint64_t)interp_shape<R>(level, (int)n, L, boxes.p, nb, in, (const R*)nullptr, false)});
     if (!CP) continue;
     if (nu && level <= 2) {
       KScope k(&tm, (t + "deposit_nu").c_str(), cells * sizeof(R));
@@ -473,9 +479,10 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
     const R n3 = (R)((float)n * (float)n * (float)n);
     if (level == 1) fft_convolve<R>(fc, mesh, cplx, G, 1, (R)0, (R)(float)n, s, &tm, tag);  // pencil_fft.f90:77
     else fft_convolve<R>(fc, mesh, cplx, G, nmesh, n3, (R)0, s, &tm, tag, fshape);        // kick.f90:846
+    const int ishape = interp_shape<R>(level, (int)n, L, boxes.p, nb, in, mesh, kph == 0);
     {
       KScope k(&tm, (t + "interp_kick").c_str(), cells * sizeof(R));
-      pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, kick_coef(in), o, s);
+      pm_interp<R>((int)g.izipx, L, boxes.p, nb, xpc(), vpc(), rhocc(), off.p, mesh, kick_coef(in), o, s, ishape);
     }
     if (nu && level <= 2) {
       KScope k(&tm, (t + "interp_kick_nu").c_str(), cells * sizeof(R));
@@ -646,6 +653,34 @@ int cube_p3m_ctx::autotune(const std::string& key, const std::vector<int>& cand,
   tune_picks[key] = pick;
   tune_file_append(key, pick);
   return pick;
+}
+
+// Block shape of a tile / sub-tile level's interpolation (cell-group form,
+// FP32), tuned on the level's first one-pass compute: each shape runs on a
+// scratch copy of the velocities (the level's real potential, kick and
+// boxes), preference 0 (8x8x16 cells, 256 threads), 1 (8x16x16, 512), 2
+// (16x16x8, 512).  Graph captures, the overlapped kick's passes and untried
+// levels otherwise use the pick so far, or shape 0; every shape gives the same
+// velocities, so the choice changes the time only.
+template <typename R>
+int cube_p3m_ctx::interp_shape(int level, int n, const PMLevel& L, const PMBox* boxes, int nb, const cube_kick_in& in,
+                               const R* phi, bool trials) {
+  if (!std::is_same<R, float>::value || level < 2 || !pm_interp_cells_form(L)) return 0;
+  const std::string key = tune_key("interp_shape", level, n);
+  const auto it = tune_picks.find(key);
+  if (it != tune_picks.end()) return it->second;
+  if (!trials || no_trials) return 0;
+  vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
+  pmtune.ensure(1);
+  return autotune(
+      key, {0, 1, 2},
+      [&](int sh) {
+        CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
+        zero_async(pmtune.p, sizeof(PMOut), s);
+        pm_interp<R>((int)g.izipx, L, boxes, nb, xpc(), vtune.p, rhocc(), off.p, phi,
+                     kick_coef(in), pmtune.p, s, sh);
+      },
+      vtune.p, sizeof(float) * 3 * (size_t)np);
 }
 
 // PP load depth (candidates in flight per lane), tuned per PP level on the

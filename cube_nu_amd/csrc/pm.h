@@ -60,10 +60,16 @@ size_t pm1_stencil_bytes(const PMLevel& L);
 // Fills xmul/xdiv from rcs and gridsize.
 void pm_level_scale(PMLevel& L);
 
+// shape: block shape of the cell-group form (0: 8x8x16 mesh cells with 256
+// threads, 1: 8x16x16 with 512, 2: 16x16x8 with 512; FP32 only, FP64 uses 0);
+// every shape gives the same result.
 template <typename R>
 void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes_dev, int nbox, const void* xp, float* vp,
                const int32_t* rhoc, const int64_t* cell_off, const R* phi, double kc, PMOut* out,
-               hipStream_t s);
+               hipStream_t s, int shape = 0);
+// True when the level's interpolation takes the cell-group form (w = 4 or 8,
+// power-of-two scale), whose block shape is tunable.
+bool pm_interp_cells_form(const PMLevel& L);
 
 // Green_3D (Green/Green.f90:1-80) evaluated in FP64 on the device, stored as
 // FP32 (n/2+1, n, n) and scaled by `factor` ((1-f_nu) or 1).

@@ -804,33 +804,35 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
 // Interpolation for the tile and sub-tile levels with w = 4 or 8 mesh cells per
 // coarse cell and a power-of-two ratio_cs/gridsize: one workgroup per group
 // of coarse cells (2x2x4 or 1x1x2) whose particles' floor(xpos) fill an
-// IBX x IBY x IBZ block of mesh cells.  For such levels xpos of a particle
+// BX x BY x BZ block of mesh cells.  For such levels xpos of a particle
 // of coarse cell i is w(i-1) - shift + (u + 1/2) 2^-sh exactly (sh = B -
 // log2 w), so floor and fraction come from the integer coordinate and the
 // block's particles are exactly those of its cells: each is read once, unlike a mesh-aligned block, which reads
 // every coarse cell its cells touch.  The staging, FD force, f2max and the
 // 27-point gather are those of k_interp.
-constexpr int IBX = 8, IBY = 8, IBZ = 16;
-constexpr int IT = 256;  // threads per block
-
-template <typename R, int IZ>
-__global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __restrict__ boxes,
+// Block shapes (BX, BY, BZ, threads), picked per level by the engine's
+// autotuner (every shape gives the same result): a larger block stages a
+// smaller share of halo (phi halo 4.2 -> 3.3 values per cell, FD force on 1.76
+// -> 1.58 cells per cell) with twice the threads.
+template <typename R, int IZ, int BX, int BY, int BZ, int NT>
+__global__ void __launch_bounds__(NT) k_interp_cells(PMLevel L, const PMBox* __restrict__ boxes,
                                                       const XT<IZ>* __restrict__ xp, float* __restrict__ vp,
                                                       const int32_t* __restrict__ rhoc,
                                                       const int64_t* __restrict__ off, const R* __restrict__ phi,
                                                       double kc, PMOut* __restrict__ out) {
   if (L.kdev) kc = *L.kdev;
-  constexpr int FX = IBX + 2, FY = IBY + 2, FZ = IBZ + 2, FN = FX * FY * FZ;
-  constexpr int PX = IBX + 6, PY = IBY + 6, PZ = IBZ + 6, PN = PX * PY * PZ;
-  constexpr int NF = (FN + IT - 1) / IT, NPH = (PN + IT - 1) / IT;
+  constexpr int FX = BX + 2, FY = BY + 2, FZ = BZ + 2, FN = FX * FY * FZ;
+  constexpr int PX = BX + 6, PY = BY + 6, PZ = BZ + 6, PN = PX * PY * PZ;
+  constexpr int NF = (FN + NT - 1) / NT, NPH = (PN + NT - 1) / NT;
   constexpr int LDSN = PN > 3 * FN ? PN : 3 * FN;
-  constexpr int MAXC = 16;
+  constexpr int MAXC = (BX / 4) * (BY / 4) * (BZ / 4);  // coarse cells of a block at w >= 4
+  constexpr int NW = NT / 64;
   __shared__ R lds[LDSN];
-  __shared__ R red[4];
-  __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[4], cxyz[MAXC];
+  __shared__ R red[NW];
+  __shared__ int cnt[MAXC], pre[MAXC + 1], wsum[NW], cxyz[MAXC];
   __shared__ int64_t gof[MAXC];
   const int w = L.w;
-  const int g[3] = {IBX / w, IBY / w, IBZ / w};  // coarse cells per block per axis
+  const int g[3] = {BX / w, BY / w, BZ / w};  // coarse cells per block per axis
   const int ncell = L.nforce / w;                // physical coarse cells of a box per axis
   const int nb[3] = {(ncell + g[0] - 1) / g[0], (ncell + g[1] - 1) / g[1], (ncell + g[2] - 1) / g[2]};
   // neighbouring blocks share phi halos: keep them on one XCD's L2
@@ -855,7 +857,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   R ph[NPH];
 #pragma unroll
   for (int q = 0; q < NPH; ++q) {
-    const int t = threadIdx.x + q * IT;
+    const int t = threadIdx.x + q * NT;
     ph[q] = (R)0;
     if (t < PN) {
       const int l[3] = {t % PX, (t / PX) % PY, t / (PX * PY)};
@@ -879,7 +881,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   }
 #pragma unroll
   for (int q = 0; q < NPH; ++q) {
-    const int t = threadIdx.x + q * IT;
+    const int t = threadIdx.x + q * NT;
     if (t < PN) lds[t] = ph[q];
   }
   __syncthreads();
@@ -892,7 +894,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   R f2 = 0;
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
-    const int t = threadIdx.x + q * IT;
+    const int t = threadIdx.x + q * NT;
     if (t < FN) {
       const int lx = t % FX, ly = (t / FX) % FY, lz = t / (FX * FY);
       const int px = lx + 2, py = ly + 2, pz = lz + 2;
@@ -924,7 +926,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
-    const int t = threadIdx.x + q * IT;
+    const int t = threadIdx.x + q * NT;
     if (t < FN) {
       lds[t] = fr[q][0];
       lds[FN + t] = fr[q][1];
@@ -936,7 +938,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
   const int sh = XP<IZ>::B - (31 - __clz(w));
   const R rsh = (R)1 / (R)(1u << sh);
   R vm[3] = {0, 0, 0};
-  for (int i = threadIdx.x; i < total; i += IT) {
+  for (int i = threadIdx.x; i < total; i += NT) {
     const int c = find_cell(pre, nC, i);
     const int64_t ip = gof[c] + (i - pre[c]);
     uint32_t uu[3];
@@ -954,7 +956,7 @@ __global__ void __launch_bounds__(IT) k_interp_cells(PMLevel L, const PMBox* __r
       // xpos = w(c-1) - s + (u + 1/2) 2^-sh exactly (sh = B - log2 w)
       f[d] = L.w * (ijk[d] - 1) - B.ushift[d] + (int)(uu[d] >> sh) - f0[d];
       tsc_weights_frac<R>(((R)(uu[d] & ((1u << sh) - 1u)) + (R)0.5) * rsh, wt[d]);
-      ok = ok && f[d] >= 0 && f[d] < (d == 0 ? IBX : (d == 1 ? IBY : IBZ));
+      ok = ok && f[d] >= 0 && f[d] < (d == 0 ? BX : (d == 1 ? BY : BZ));
     }
     if (!ok) continue;  // unreachable for a power-of-two xmul; guards the LDS reads
     R dv[3] = {0, 0, 0};
@@ -1180,17 +1182,35 @@ void launch_interp_bd(const PMLevel& L, const PMBox* boxes, int nbox, const void
   CUBE_HIP(hipGetLastError());
 }
 
+template <typename R, int IZ, int BX, int BY, int BZ, int NT>
+void launch_interp_cells(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp,
+                         const int32_t* rhoc, const int64_t* off, const R* phi, double kc, PMOut* out,
+                         hipStream_t s) {
+  const int ncell = L.nforce / L.w;
+  int64_t grid = nbox;
+  const int g[3] = {BX / L.w, BY / L.w, BZ / L.w};
+  for (int d = 0; d < 3; ++d) grid *= (ncell + g[d] - 1) / g[d];
+  hipLaunchKernelGGL((k_interp_cells<R, IZ, BX, BY, BZ, NT>), dim3((unsigned)grid), dim3(NT), 0, s, L, boxes,
+                     (const XT<IZ>*)xp, vp, rhoc, off, phi, kc, out);
+  CUBE_HIP(hipGetLastError());
+}
+
 template <typename R, int IZ>
 void launch_interp(const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp, const int32_t* rhoc,
-                   const int64_t* off, const R* phi, double kc, PMOut* out, hipStream_t s) {
-  if (!L.periodic && !L.xdiv && (L.w == 4 || L.w == 8) && L.nforce % L.w == 0) {
-    const int ncell = L.nforce / L.w;
-    int64_t grid = nbox;
-    const int g[3] = {IBX / L.w, IBY / L.w, IBZ / L.w};
-    for (int d = 0; d < 3; ++d) grid *= (ncell + g[d] - 1) / g[d];
-    hipLaunchKernelGGL((k_interp_cells<R, IZ>), dim3((unsigned)grid), dim3(IT), 0, s, L, boxes, (const XT<IZ>*)xp,
-                       vp, rhoc, off, phi, kc, out);
-    CUBE_HIP(hipGetLastError());
+                   const int64_t* off, const R* phi, double kc, PMOut* out, hipStream_t s, int shape) {
+  if (pm_interp_cells_form(L)) {
+    // the larger shapes' LDS (FP32: 38.9 KB) exceeds the static limit in FP64
+    if constexpr (std::is_same<R, float>::value) {
+      if (shape == 1) {
+        launch_interp_cells<R, IZ, 8, 16, 16, 512>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
+        return;
+      }
+      if (shape == 2) {
+        launch_interp_cells<R, IZ, 16, 16, 8, 512>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
+        return;
+      }
+    }
+    launch_interp_cells<R, IZ, 8, 8, 16, 256>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
     return;
   }
   launch_interp_bd<R, IZ, 8>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
@@ -1310,9 +1330,13 @@ void pm1_deposit(int izipx, const PMLevel& L, const void* xp, const int32_t* rho
 template <typename R>
 void pm_interp(int izipx, const PMLevel& L, const PMBox* boxes, int nbox, const void* xp, float* vp,
                const int32_t* rhoc, const int64_t* off, const R* phi, double kc, PMOut* out,
-               hipStream_t s) {
-  if (izipx == 2) launch_interp<R, 2>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
-  else launch_interp<R, 1>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s);
+               hipStream_t s, int shape) {
+  if (izipx == 2) launch_interp<R, 2>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s, shape);
+  else launch_interp<R, 1>(L, boxes, nbox, xp, vp, rhoc, off, phi, kc, out, s, shape);
+}
+
+bool pm_interp_cells_form(const PMLevel& L) {
+  return !L.periodic && !L.xdiv && (L.w == 4 || L.w == 8) && L.nforce % L.w == 0;
 }
 
 template void pm_deposit<float>(int, const PMLevel&, const PMBox*, int, const void*, const int32_t*, const int64_t*,
@@ -1324,9 +1348,9 @@ template void pm1_deposit<float>(int, const PMLevel&, const void*, const int32_t
 template void pm1_deposit<double>(int, const PMLevel&, const void*, const int32_t*, const int64_t*, void*, double*,
                                   float, PMOut*, hipStream_t);
 template void pm_interp<float>(int, const PMLevel&, const PMBox*, int, const void*, float*, const int32_t*,
-                               const int64_t*, const float*, double, PMOut*, hipStream_t);
+                               const int64_t*, const float*, double, PMOut*, hipStream_t, int);
 template void pm_interp<double>(int, const PMLevel&, const PMBox*, int, const void*, float*, const int32_t*,
-                                const int64_t*, const double*, double, PMOut*, hipStream_t);
+                                const int64_t*, const double*, double, PMOut*, hipStream_t, int);
 
 void green_device(int n, float apm_soft, float apm_range, float gridsize, double factor, float* out, hipStream_t s) {
   int64_t tot = (int64_t)(n / 2 + 1) * n * n;

@@ -693,7 +693,7 @@ int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, const PPBox* boxes, int nb
   vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
   otune.ensure(1);
   return autotune(
-      tune_key("pp_depth", iapm, L.npgrid), {6, 4, 8},
+      tune_key("pp_depth", iapm, L.npgrid), {6, 4, 8, kPPBrick8 + 6, kPPBrick8 + 4, kPPBrick8 + 8},
       [&](int u) {
         CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
         zero_async(otune.p, sizeof(PPOut), s);

@@ -54,8 +54,10 @@ int64_t pp_bin_count(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nb
                      const int32_t* rhoc, const int64_t* cell_off, PPScratch& S, hipStream_t s);
 void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes_dev, int nbox, const void* xp, const int32_t* rhoc,
                  const int64_t* cell_off, PPScratch& S, hipStream_t s);
-// unroll: candidate loads in flight per lane in the FP32 kernel (4, 6 or 8);
-// every value gives the same result.
+// unroll: candidate loads in flight per lane in the FP32 kernel (4, 6 or 8),
+// plus kPPBrick8 for bricks of 8^3 PP cells instead of 4^3; every value gives
+// the same result.
+constexpr int kPPBrick8 = 100;
 void pp_force(const PPLevel& L, int precision, const PPBox* boxes_dev, int nbox, float* vp, PPScratch& S, PPOut* out,
               hipStream_t s, int unroll = 6);
 

@@ -124,7 +124,9 @@ __global__ void __launch_bounds__(256) k_pp_bin(PPLevel L, const PPBox* __restri
   }
 }
 
-constexpr int PB = 4;       // physical PP cells per brick per dim
+// Physical PP cells per brick per dim: PB = 4 or 8, chosen with the load
+// depth by the engine's autotuner (a target's sums do not depend on the
+// brick it is in, so neither choice changes a result).
 // Lanes per target in the FP32 force kernel: a target's candidates are dealt
 // to kPPGroup lanes in turn and their partial sums combined at the end (26.1
 // ms at 512^3 against 27.3 with one lane per target, 27.2 with four, 31.5
@@ -279,7 +281,7 @@ constexpr int pp_min_blocks_per_cu() {
 }
 constexpr int kPPClaimCounters = 8;
 
-template <typename R, int U, int G>
+template <typename R, int U, int G, int PB>
 __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   int64_t nbricks, const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
@@ -502,27 +504,29 @@ void pp_bin_fill(int izipx, const PPLevel& L, const PPBox* boxes, int nbox, cons
   CUBE_HIP(hipGetLastError());
 }
 
-template <int U, int G>
+template <int U, int G, int PB>
 static void launch_pp_f32(dim3 grid, hipStream_t s, const PPLevel& L, const PPBox* boxes, int64_t nbricks,
                           PPScratch& S, float* vp, PPOut* out) {
   static_assert((U - 1) * G < kPPPad, "reads past a run stay inside the entry padding");
-  hipLaunchKernelGGL((k_pp_force<float, U, G>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries, vp,
-                     out, S.claim);
+  hipLaunchKernelGGL((k_pp_force<float, U, G, PB>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries,
+                     vp, out, S.claim);
 }
 
-template <int G>
-static void launch_pp_f32_u(int unroll, dim3 grid, hipStream_t s, const PPLevel& L, const PPBox* boxes,
+template <int G, int PB>
+static void launch_pp_f32_u(int depth, dim3 grid, hipStream_t s, const PPLevel& L, const PPBox* boxes,
                             int64_t nbricks, PPScratch& S, float* vp, PPOut* out) {
-  switch (unroll) {
-    case 4: launch_pp_f32<4, G>(grid, s, L, boxes, nbricks, S, vp, out); break;
-    case 8: launch_pp_f32<8, G>(grid, s, L, boxes, nbricks, S, vp, out); break;
-    default: launch_pp_f32<6, G>(grid, s, L, boxes, nbricks, S, vp, out); break;
+  switch (depth) {
+    case 4: launch_pp_f32<4, G, PB>(grid, s, L, boxes, nbricks, S, vp, out); break;
+    case 8: launch_pp_f32<8, G, PB>(grid, s, L, boxes, nbricks, S, vp, out); break;
+    default: launch_pp_f32<6, G, PB>(grid, s, L, boxes, nbricks, S, vp, out); break;
   }
 }
 
 void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, float* vp, PPScratch& S, PPOut* out,
               hipStream_t s, int unroll) {
-  const int64_t nbr = (L.npgrid + PB - 1) / PB;
+  const int pb = precision == 32 && unroll >= kPPBrick8 ? 8 : 4;
+  const int depth = unroll % kPPBrick8;
+  const int64_t nbr = (L.npgrid + pb - 1) / pb;
   const int64_t nbricks = nbr * nbr * nbr * nbox;
   static int n_cu = 0;
   if (n_cu == 0) {
@@ -538,10 +542,12 @@ void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, flo
   const int per_cu = precision == 64 ? 4 : pp_min_blocks_per_cu<float>();
   const dim3 grid((unsigned)std::min<int64_t>(nbricks, (int64_t)n_cu * per_cu));
   if (precision == 64)
-    hipLaunchKernelGGL((k_pp_force<double, 6, 1>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries,
-                       vp, out, S.claim);
+    hipLaunchKernelGGL((k_pp_force<double, 6, 1, 4>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start,
+                       S.entries, vp, out, S.claim);
+  else if (pb == 8)
+    launch_pp_f32_u<kPPGroup, 8>(depth, grid, s, L, boxes, nbricks, S, vp, out);
   else
-    launch_pp_f32_u<kPPGroup>(unroll, grid, s, L, boxes, nbricks, S, vp, out);
+    launch_pp_f32_u<kPPGroup, 4>(depth, grid, s, L, boxes, nbricks, S, vp, out);
   CUBE_HIP(hipGetLastError());
 }
 

@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
                                                   PPOut* __restrict__ out, unsigned* __restrict__ claim) {
   constexpr int NB = PB + 2, RW = kSX * NB + 1;  // staged sub-bin starts per (y, z) row
   __shared__ int pre[PB * PB * PB + 1];
-  __shared__ int64_t rs[NB * NB * RW];
+  __shared__ int rs[NB * NB * RW];  // entry indices (a batch holds < 2^31 entries)
   __shared__ float red[4];
   __shared__ unsigned long long cnt_red[2];
   __shared__ float4 qtab[kQSeg];
@@ -331,10 +331,10 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
     // outside the PP grid are empty, sub-bins past the row end read the row end
     for (int q = threadIdx.x; q < NB * NB * RW; q += blockDim.x) {
       const int xs = q % RW, y = c0[1] - 1 + (q / RW) % NB, z = c0[2] - 1 + q / (RW * NB);
-      int64_t st = 0;
+      int st = 0;
       if (y <= L.npgrid + 1 && z <= L.npgrid + 1) {
         const int64_t row = (cbase + ((int64_t)z * m + y) * m) * kSX;
-        st = start[row + min((int64_t)kSX * (c0[0] - 1) + xs, mS)];
+        st = (int)start[row + min((int64_t)kSX * (c0[0] - 1) + xs, mS)];
       }
       rs[q] = st;
     }
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
           const int ny = c0[1] + ly + dy, nz = c0[2] + lz + dz;
           const bool gyz = ny == 0 || nz == 0 || ny == L.npgrid + 1 || nz == L.npgrid + 1;
           const int row = rowt + (dz * NB + dy) * RW;
-          const int64_t beg = rs[row + wlo];
+          const int beg = rs[row + wlo];
           const int nn = (int)(rs[row + whi + 1] - beg);
           // candidates [ph0, ph1) of the window are physical: the ghost cells
           // x-1 / x+1 lie before / after them, a ghost row holds none

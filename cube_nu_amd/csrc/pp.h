@@ -6,8 +6,8 @@ namespace cube {
 
 // PP cells are split into kSX sub-bins along x in the binned CSR, so that the
 // partners of a target along a (dy, dz) column are one short contiguous run
-// (8 sub-bins: 25.4 ms at 512^3 against 26.1 with 4, the window's edges
-// cutting closer to x +- r_pp).
+// (8 sub-bins: 25.4 ms at 512^3 against 26.1 with 4 and 25.8 with 16, the
+// window's edges cutting closer to x +- r_pp against a larger staged table).
 constexpr int kSX = 8;
 
 struct PPLevel {

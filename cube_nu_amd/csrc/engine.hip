@@ -547,7 +547,6 @@ void cube_p3m_ctx::pp_phase(const std::vector<int32_t>& ires_h, const std::vecto
       // r_pp in PP-cell units plus a margin of 1e-3 cell, far above the FP32
       // rounding of the coordinates, so the windows never cut a pair in range
       const double rc = (double)L.pp_range * L.sf / L.rcs + 1e-3;
-      L.rwin = (float)(kSX * rc);
       L.rcut2 = (float)(rc * rc);
     }
     const int64_t m = L.npgrid + 2, m3 = m * m * m * kSX;

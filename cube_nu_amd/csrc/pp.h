@@ -23,7 +23,6 @@ struct PPLevel {
   double v_scale;  // a_mid*dt/6/pi in FP64 (kick.f90:574 forms it in real(4))
   const double* vdev = nullptr;  // v_scale in device memory, read in its place when set
   float inv_rcs;   // 1/rcs: fine -> coarse units (exact for a power-of-two rcs)
-  float rwin;      // kSX * (r_pp in PP-cell units + margin): x window half-width in sub-bins
   float rcut2;     // (r_pp in PP-cell units + margin)^2: y/z column cut
 };
 

@@ -374,10 +374,8 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
         const float xv = (float)L.sf * (xme[d] * L.inv_rcs - xoff[d]);
         fr[d] = xv - (float)(c0[d] - 1 + (d == 0 ? lx : (d == 1 ? ly : lz)));  // offset in its PP cell
       }
-      // x window in local sub-bins: cells lx .. lx+2 of the staged rows
+      // the target's x in local sub-bins (cells lx .. lx+2 of the staged rows)
       const float u = (fr[0] + (float)(lx + 1)) * (float)kSX;
-      const int wlo = max((int)floorf(u - L.rwin), kSX * lx);
-      const int whi = min((int)floorf(u + L.rwin), kSX * (lx + 3) - 1);
       R f1[3] = {0, 0, 0};
       // pairs in range of this target, and those with a ghost partner (32-bit)
       int npair = 0, nghost = 0;
@@ -386,7 +384,13 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
         for (int dy = -1; dy <= 1; ++dy) {
           const float ey = dy < 0 ? fr[1] : (dy > 0 ? 1.0f - fr[1] : 0.0f);
           const float ez = dz < 0 ? fr[2] : (dz > 0 ? 1.0f - fr[2] : 0.0f);
-          if (ey * ey + ez * ez >= L.rcut2) continue;
+          const float e2 = ey * ey + ez * ez;
+          if (e2 >= L.rcut2) continue;
+          // x window of the column: a partner at y/z distance >= (ey, ez)
+          // is in range only within x +- sqrt(r_pp^2 - ey^2 - ez^2)
+          const float hw = (float)kSX * sqrtf(L.rcut2 - e2);
+          const int wlo = max((int)floorf(u - hw), kSX * lx);
+          const int whi = min((int)floorf(u + hw), kSX * (lx + 3) - 1);
           const int ny = c0[1] + ly + dy, nz = c0[2] + lz + dz;
           const bool gyz = ny == 0 || nz == 0 || ny == L.npgrid + 1 || nz == L.npgrid + 1;
           const int row = rowt + (dz * NB + dy) * RW;

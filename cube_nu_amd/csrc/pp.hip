@@ -238,8 +238,9 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // Persistent: each workgroup claims bricks of PB^3 physical PP cells from one
 // of kPPClaimCounters counters (workgroup i uses counter i % 8, which hands
 // out the bricks of sub-tiles i % 8, i % 8 + 8, ..., one sub-tile at a time,
-// so that a sub-tile's bricks run on one XCD and share its L2) until they run
-// out.  The host orders the boxes
+// so that a sub-tile's bricks run on one XCD and share its L2; with fewer
+// than 64 sub-tiles in a launch, bricks 8c + i % 8, so that every counter has
+// work) until they run out.  The host orders the boxes
 // densest sub-tile first (kick.f90:132-138 sorts by npc_max for
 // schedule(dynamic,1), kick.f90:285), so every counter drains the heaviest
 // bricks first and the launch does not end on a few heavy workgroups.  In a
@@ -285,7 +286,8 @@ template <typename R, int U, int G, int PB>
 __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPLevel L, const PPBox* __restrict__ boxes,
                                                   int64_t nbricks, const int64_t* __restrict__ start,
                                                   const float4* __restrict__ ent, float* __restrict__ vp,
-                                                  PPOut* __restrict__ out, unsigned* __restrict__ claim) {
+                                                  PPOut* __restrict__ out, unsigned* __restrict__ claim,
+                                                  bool xcd_local) {
   constexpr int NB = PB + 2, RW = kSX * NB + 1;  // staged sub-bin starts per (y, z) row
   __shared__ int pre[PB * PB * PB + 1];
   __shared__ int rs[NB * NB * RW];  // entry indices (a batch holds < 2^31 entries)
@@ -306,13 +308,18 @@ __global__ void __launch_bounds__(256, pp_min_blocks_per_cu<R>()) k_pp_force(PPL
   for (;;) {
     __syncthreads();  // the previous brick's tables are no longer read
     if (threadIdx.x == 0) {
-      // counter ctr hands out the bricks of sub-tiles ctr, ctr + 8, ... (in
-      // the host's densest-first order), one sub-tile's bricks in turn: its
-      // workgroups (blockIdx % 8 == ctr) share an XCD, whose L2 then holds
-      // the neighbourhoods that a sub-tile's bricks read in common
       const int64_t k = (int64_t)atomicAdd(&claim[ctr], 1u), nbps = (int64_t)nbr * nbr * nbr;
-      const int64_t box = (k / nbps) * kPPClaimCounters + ctr;
-      s_brick = box < nbricks / nbps ? box * nbps + k % nbps : nbricks;
+      if (xcd_local) {
+        // counter ctr hands out the bricks of sub-tiles ctr, ctr + 8, ...
+        // (in the host's densest-first order), one sub-tile's bricks in
+        // turn: its workgroups (blockIdx % 8 == ctr) share an XCD, whose L2
+        // then holds the neighbourhoods that a sub-tile's bricks read in
+        // common
+        const int64_t box = (k / nbps) * kPPClaimCounters + ctr;
+        s_brick = box < nbricks / nbps ? box * nbps + k % nbps : nbricks;
+      } else {
+        s_brick = k * kPPClaimCounters + ctr;  // few sub-tiles: bricks 8k + ctr, every counter busy
+      }
     }
     __syncthreads();
     int64_t b = s_brick;
@@ -512,8 +519,12 @@ template <int U, int G, int PB>
 static void launch_pp_f32(dim3 grid, hipStream_t s, const PPLevel& L, const PPBox* boxes, int64_t nbricks,
                           PPScratch& S, float* vp, PPOut* out) {
   static_assert((U - 1) * G < kPPPad, "reads past a run stay inside the entry padding");
+  // whole sub-tiles per counter only when there are enough of them to keep
+  // the eight counters evenly busy
+  const bool xcd_local = nbricks / ((int64_t)((L.npgrid + PB - 1) / PB) * ((L.npgrid + PB - 1) / PB) *
+                                    ((L.npgrid + PB - 1) / PB)) >= 64;
   hipLaunchKernelGGL((k_pp_force<float, U, G, PB>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start, S.entries,
-                     vp, out, S.claim);
+                     vp, out, S.claim, xcd_local);
 }
 
 template <int G, int PB>
@@ -547,7 +558,7 @@ void pp_force(const PPLevel& L, int precision, const PPBox* boxes, int nbox, flo
   const dim3 grid((unsigned)std::min<int64_t>(nbricks, (int64_t)n_cu * per_cu));
   if (precision == 64)
     hipLaunchKernelGGL((k_pp_force<double, 6, 1, 4>), grid, dim3(256), 0, s, L, boxes, nbricks, S.start,
-                       S.entries, vp, out, S.claim);
+                       S.entries, vp, out, S.claim, nbox >= 64);
   else if (pb == 8)
     launch_pp_f32_u<kPPGroup, 8>(depth, grid, s, L, boxes, nbricks, S, vp, out);
   else

@@ -175,6 +175,18 @@ def test_cfg1_128_ten_fixed_steps():
             assert s0 == 0 and oo["nplocal"] == d["nplocal"] == ng ** 3 and oo["nlayer"] == d["nlayer"]
             xb, vb, pb, rb = orc.buffer(P, xo, vo, po, ro, cap)
             assert np.array_equal(rb, rg) and np.array_equal(xb, xg) and np.array_equal(pb, pg)
+            _log(f"cfg1 step {n + 1}: drift and buffer bit-exact")
+            if n not in (0, 1, nsteps // 2, nsteps - 1):
+                # the oracle's kick (tens of seconds of CPU at 128^3) on four
+                # of the ten steps; drift and buffer are checked on every one
+                s1, x1, v1, p1, r1, _ = orc.drift(P, *own, dt_old, dt, vz_o)
+                assert s1 == 0
+                x1, v1, p1, r1 = orc.buffer(P, x1, v1, p1, r1, cap)
+                k1, _, _ = orc.kick(P, x1, v1, r1, tfs, a_mid, dt)
+                orc.buffer_v(P, v1, r1)
+                own = (x1, v1, p1, r1)
+                vz_g, vz_o, dt_old = k["vz_max"], k1["vz_max"], dt
+                continue
             v_before = vb.copy()
             ko, _, _ = orc.kick(P, xb, vb, rb, tfs, a_mid, dt)
             orc.buffer_v(P, vb, rb)

@@ -477,7 +477,11 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
       for (int64_t i = 0; i < nreal; ++i) density_host[i] = (float)tmp[i];
     }
     const R n3 = (R)((float)n * (float)n * (float)n);
-    if (level == 1) fft_convolve<R>(fc, mesh, cplx, G, 1, (R)0, (R)(float)n, s, &tm, tag);  // pencil_fft.f90:77
+    // PM1 (pencil_fft.f90:77) on the generic line passes: under the overlapped
+    // kick a coarse-mesh plane pass (one 1024-thread, 67.5 KB workgroup per
+    // z plane) waits for room on the CUs behind the second stream's list
+    // building, 1.9 ms against 0.34 for the line passes
+    if (level == 1) fft_convolve<R>(fc, mesh, cplx, G, 1, (R)0, (R)(float)n, s, &tm, tag, -1);
     else fft_convolve<R>(fc, mesh, cplx, G, nmesh, n3, (R)0, s, &tm, tag, fshape);        // kick.f90:846
     const int ishape = interp_shape<R>(level, (int)n, L, boxes.p, nb, in, mesh, kph == 0);
     {

@@ -33,7 +33,8 @@ struct FFTCube {
 // nbox * n*n*(n/2+1) elements.  The division follows the reference: one FP
 // division by div3 when div3 != 0, else three divisions by div1.
 // shape: launch shape of the size-specialised passes (0, 1 or 2; the same
-// result for each), picked by the caller's autotuner.
+// result for each), picked by the caller's autotuner; -1 selects the generic
+// line passes for every size.
 template <typename R>
 void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox, R div3,
                   R div1, hipStream_t s, KTimer* tm = nullptr, const char* tag = "", int shape = 0);

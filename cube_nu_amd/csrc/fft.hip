@@ -489,7 +489,7 @@ template <typename R>
 void fft_convolve(const FFTCube<R>& c, R* rho, typename Cplx<R>::T* work, const float* G, int64_t nbox, R div3,
                   R div1, hipStream_t s, KTimer* tm, const char* tag, int shape) {
   using C = typename Cplx<R>::T;
-  if (use_plane_path() &&
+  if (use_plane_path() && shape >= 0 &&
       fft_convolve_plane<R>(c, rho, work, G, nbox, div3, div1, s, tm ? tm : &null_timer(), tag, shape))
     return;
   const int n = c.n, h = n / 2;

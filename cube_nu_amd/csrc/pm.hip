@@ -518,9 +518,15 @@ __global__ void __launch_bounds__(128) k_pm1_stencil(PMLevel L, const XT<IZ>* __
   Acc a27[27];
 #pragma unroll
   for (int q = 0; q < 27; ++q) a27[q] = 0;
+  // one mesh cell per coarse cell: xpos - floor(xpos) = (u + 1/2) 2^-B exactly
+  const bool unit = !L.xdiv && L.xmul == 1.0;
+  constexpr R kRes = (R)1 / (R)(1u << XP<IZ>::B);
   for (int l = lane; l < n; l += 64) {
     R w[3][3];
-    for (int d = 0; d < 3; ++d) tsc_weights<R>(pm_xpos(L, ijk[d], XP<IZ>::off(xp, base + l, d), 0), w[d]);
+    for (int d = 0; d < 3; ++d) {
+      if (unit) tsc_weights_frac<R>(((R)XP<IZ>::raw(xp, base + l, d) + (R)0.5) * kRes, w[d]);
+      else tsc_weights<R>(pm_xpos(L, ijk[d], XP<IZ>::off(xp, base + l, d), 0), w[d]);
+    }
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       const R cw = (w[0][q % 3] * w[1][(q / 3) % 3]) * w[2][q / 9];
@@ -738,6 +744,7 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
   __syncthreads();
   // 3. particles whose floor(xpos) lies in this block
   constexpr double kRes = 1.0 / (double)(1u << XP<IZ>::B);
+  const bool unit = !L.xdiv && L.xmul == 1.0;  // one mesh cell per coarse cell (PM1)
   R vm[3] = {0, 0, 0};
   for (int i0 = threadIdx.x; i0 < total; i0 += 2 * 256) {
 #pragma unroll
@@ -767,8 +774,15 @@ __global__ void __launch_bounds__(256) k_interp(PMLevel L, const PMBox* __restri
       int f[3];
       bool mine = true;
       for (int d = 0; d < 3; ++d) {
-        const double xpos = pm_xpos(L, ijk[d], ((double)uu[d] + 0.5) * kRes, B.ushift[d]);
-        f[d] = tsc_weights<R>(xpos, w[d]) - f0[d];
+        if (unit) {
+          // xpos = (c-1) + (u + 1/2) 2^-B - s exactly: floor c-1-s, fraction
+          // (u + 1/2) 2^-B (exact in R), the FP64 path's values
+          f[d] = ijk[d] - 1 - B.ushift[d] - f0[d];
+          tsc_weights_frac<R>(((R)uu[d] + (R)0.5) * (R)kRes, w[d]);
+        } else {
+          const double xpos = pm_xpos(L, ijk[d], ((double)uu[d] + 0.5) * kRes, B.ushift[d]);
+          f[d] = tsc_weights<R>(xpos, w[d]) - f0[d];
+        }
         mine = mine && f[d] >= 0 && f[d] < BD;
       }
       if (!mine) continue;

@@ -11,6 +11,7 @@ struct DriftParams {
   double dt_mid;  // (dt_old + dt)/2 in FP64 (the reference's real(4) coefficient, kept FP64 here)
   int nlayer;
   int nt, nte, ncb, nnt, rcs, izipx;
+  double inv_rcs;  // 1/rcs, exact (rcs is a power of two), so x * inv_rcs == x / rcs
   int M[3];       // maximum |cell shift| per axis
   int D[3];       // 2M+1
 };

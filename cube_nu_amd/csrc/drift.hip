@@ -37,7 +37,7 @@ __device__ inline void dest_cell(const DriftParams& P, const XT<IZ>* xp, const f
                                  const int ijk[3], int g[3]) {
   for (int d = 0; d < 3; ++d) {
     const double wv = P.dt_mid * (double)vp[3 * ip + d];        // FP64 coefficient
-    const double dx = wv / (double)P.rcs;                        // drift.f90:57
+    const double dx = wv * P.inv_rcs;                            // drift.f90:57 (exact: rcs = 2^k)
     const double xq = ((double)ijk[d] - 1.0) + XP<IZ>::off(xp, ip, d);  // drift.f90:59
     g[d] = (int)ceil(xq + dx);                                   // drift.f90:63
   }
@@ -254,7 +254,8 @@ __global__ void k_drift_scatter(DriftParams P, const XT<IZ>* __restrict__ xp, co
   const int cnt = rhoc[c];
   const int64_t base = off[c];
   const int Dt = P.D[0] * P.D[1] * P.D[2];
-  const double xr = 1.0 / (double)(1 << XP<IZ>::B);
+  // 1 / (x_resolution rcs): both powers of two, so wv * shs == wv / (xr rcs)
+  const double shs = P.inv_rcs * (double)(1 << XP<IZ>::B);
   for (int l = lane; l < cnt; l += 64) {
     const int64_t ip = base + l;
     const int r = rank[ip];
@@ -264,7 +265,7 @@ __global__ void k_drift_scatter(DriftParams P, const XT<IZ>* __restrict__ xp, co
     const int64_t pos = base_out[pcell(P, T, g)] + table[cl * Dt + dcode(P, g, ijk)] + r;
     for (int d = 0; d < 3; ++d) {
       const double wv = P.dt_mid * (double)vp[3 * ip + d];
-      const double sh = wv / (xr * (double)P.rcs);
+      const double sh = wv * shs;  // drift.f90:117
       const long long dn = (long long)round(sh);  // nint: half away from zero
       const long long nv = (long long)xp[3 * ip + d] + dn;
       if (IZ == 2) xo[3 * pos + d] = (int16_t)(uint16_t)(nv & 0xffff);

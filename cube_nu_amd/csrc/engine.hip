@@ -1076,6 +1076,7 @@ void cube_p3m_ctx::drift(const cube_drift_in& in, cube_drift_out& out) {
   P.dt_mid = (in.dt_old + in.dt) / 2.0;                                                     // drift.f90:21
   P.nlayer = 2 * (int)std::ceil((P.dt_mid * (double)in.vz_max) / (double)g.rcs) + 2;        // drift.f90:26
   P.nt = (int)g.nt; P.nte = (int)g.nte; P.ncb = (int)g.ncb; P.nnt = (int)g.nnt; P.rcs = (int)g.rcs;
+  P.inv_rcs = 1.0 / (double)g.rcs;
   P.izipx = (int)g.izipx;
   out.nlayer = P.nlayer;
   const int64_t nt3 = g.nt * g.nt * g.nt, nphys_cells = g.ntiles * nt3;

@@ -324,6 +324,10 @@ void cube_p3m_ctx::init() {
   prec = c.precision;
   g.init(c.ng, c.nnt, c.nns, c.ratio_cs, c.ncb, c.ngb, c.izipx, c.image_buffer, c.tile_buffer);
   if (g.ngb < 4) throw Error(CUBE_ERR_ARG, "ngb must be >= 4 (gradient stencil)");
+  // the drift divides by ratio_cs as a product with its reciprocal and the PP
+  // binning takes xvec = xf / ratio_cs: both exact only for a power of two
+  if (c.ratio_cs < 1 || (c.ratio_cs & (c.ratio_cs - 1)) != 0)
+    throw Error(CUBE_ERR_ARG, "ratio_cs must be a power of two");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(CUBE_ERR_NO_DEVICE, "no HIP device visible");
   CUBE_HIP(hipGetDevice(&device));

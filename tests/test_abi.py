@@ -44,6 +44,17 @@ def test_create_without_device_or_bad_config():
     assert rc == CUBE_ERR_NO_DEVICE
 
 
+def test_create_rejects_ratio_cs_not_a_power_of_two():
+    """The drift's division by ratio_cs is a product with its reciprocal and
+    the PP binning takes xvec = xf / ratio_cs: both exact only for a power of
+    two, so another ratio_cs is refused with CUBE_ERR_ARG before any device
+    call."""
+    lib = load_library()
+    ctx = ctypes.c_void_p()
+    cfg = Config(96, 4, 4, 3, 4, 16, 2, 32, 600.0, 3.5, 3.5, 3.5, 0.06, 1, 1, 1, 0.0, 0.0, 2.0, 6.0, 0)
+    assert lib.cube_p3m_create(ctypes.byref(cfg), ctypes.byref(ctx)) == 1
+
+
 DRIVER = os.path.join(ROOT, "cube_nu_amd", "lib", "cube_p3m_driver")
 
 

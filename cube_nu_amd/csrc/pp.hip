@@ -250,9 +250,9 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // are added by a fixed butterfly, so the result is deterministic.  PP cells
 // are split into kSX sub-bins along x and
 // the CSR is x-ordered along each (y, z) row, so the part of a (dy, dz)
-// column that can hold partners, x in [x_t - r_pp, x_t + r_pp], is one
-// contiguous run found from two sub-bin starts; columns whose y/z distance
-// from the target exceeds r_pp are skipped.  Both windows carry a margin far
+// column that can hold partners, x within x_t +- sqrt(r_pp^2 - e^2) for the
+// column's nearest y/z distance e from the target, is one contiguous run
+// found from two sub-bin starts; columns with e > r_pp are skipped.  Both windows carry a margin far
 // above FP32 rounding, so every pair the reference tests in range is tested
 // here with the reference's FP32 operations and the interacting set is the
 // reference's.  The sub-bin starts of the (PB+2)^3 neighbourhood are staged

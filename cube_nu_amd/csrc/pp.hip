@@ -276,9 +276,10 @@ __device__ inline void pair_force<double>(const PPLevel& L, const float4*, const
 // value), i.e. at most 64 VGPRs and eight wavefronts per SIMD to cover the
 // candidate loads' latency (28.5 vs 30.2 ms at 512^3 with 71 VGPRs and seven
 // wavefronts, a few values spilled); the FP64 form keeps its registers.  The
-// 8^3-brick variants' staged starts (32 KB) leave room for four workgroups
-// only, so there the target is unmet (the compiler says so) while the
-// 64-VGPR cap still holds; they are autotune candidates that lose at 512^3.
+// 8^3-brick variants' staged starts (35.5 KB of LDS per workgroup) leave room
+// for four workgroups only, so there the target is unmet (the compiler says
+// so) and they take 88 VGPRs without spills; they are autotune candidates
+// that lose at 512^3.
 template <typename R>
 constexpr int pp_min_blocks_per_cu() {
   return sizeof(R) == 4 ? 8 : 1;

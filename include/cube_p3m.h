@@ -213,9 +213,11 @@ int cube_p3m_make_ic(cube_p3m_ctx* ctx, const cube_ic_params* ic,
 int cube_p3m_cicpower(cube_p3m_ctx* ctx, int32_t nw, float* xi, int32_t nbin,
                       float* delta, double* mean);
 
-/* ---- measurement -------------------------------------------------------- */
+/* Wait until the context's work has finished (the kick's second stream is
+ * joined into the context stream, so this waits for it too). */
 int cube_p3m_synchronize(cube_p3m_ctx* ctx);
-/* Enable HIP-event timing of every kernel launch on the context stream. */
+
+/* ---- neutrino particles ------------------------------------------------- */
 /* Kick with a second, PM-only species (neutrino particles) resident in
  * another context of the same geometry, coordinates and precision (the
  * buffered layout, after its own buffer step).  Both species deposit into
@@ -239,6 +241,7 @@ int cube_p3m_kick_resident_nu(cube_p3m_ctx* ctx, cube_p3m_ctx* nu,
  * the largest |v_z| afterwards (the next drift's bound).  Needs IDs. */
 int cube_p3m_add_thermal_velocities(cube_p3m_ctx* ctx, double v_scale,
                                     uint64_t seed, float* vz_max);
+/* ---- graph capture and measurement -------------------------------------- */
 /* Graph mode for full resident kicks (and the kick + buffer_v of
  * cube_p3m_step_resident): the kick's compute passes -- PM1 + PM2, then
  * PM3 + PP (+ buffer_v), whose lists are built on a second stream meanwhile
@@ -250,6 +253,7 @@ int cube_p3m_add_thermal_velocities(cube_p3m_ctx* ctx, double v_scale,
  * graph-run kicks (graph_captures, graph_launches).  No reference
  * counterpart (the reference's step loop is main.f90:30-59). */
 int cube_p3m_graph_enable(cube_p3m_ctx* ctx, int32_t enable);
+/* Enable HIP-event timing of every kernel launch on the context stream. */
 int cube_p3m_timing_enable(cube_p3m_ctx* ctx, int32_t enable);
 int cube_p3m_timing_reset(cube_p3m_ctx* ctx);
 /* Text report, one line per kernel: "name calls total_ms bytes flops". */

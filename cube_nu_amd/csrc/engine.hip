@@ -78,10 +78,30 @@ inline uint64_t fbits(float f) {
 
 __global__ void k_set_f64(double* p, double v) { *p = v; }
 
+// Candidates of each tuned choice, in order of preference.
+const std::vector<int> kTunePPDepth = {6, 4, 8, kPPBrick8 + 6, kPPBrick8 + 4, kPPBrick8 + 8};
+const std::vector<int> kTuneDepListThreads = {512, 256, 1024};
+const std::vector<int> kTuneFFTShape = {0, 1, 2};
+const std::vector<int> kTuneInterpShape = {0, 1, 2};
+
 // Autotuner picks persisted across runs: when the environment names a file in
 // CUBE_AUTOTUNE_FILE, a context reads the picks recorded there (lines
 // "key pick") and appends every pick it makes, so later runs skip the trials.
+// A line whose key names no tuned choice, or whose pick is not one of that
+// choice's candidates (a file from another build, or edited), is ignored.
 std::mutex g_tune_file_mu;
+
+bool tune_pick_valid(const std::string& key, int pick) {
+  static const std::pair<const char*, const std::vector<int>*> choices[] = {
+      {"pp_depth_l", &kTunePPDepth},
+      {"dep_list_threads_l", &kTuneDepListThreads},
+      {"fft_shape_l", &kTuneFFTShape},
+      {"interp_shape_l", &kTuneInterpShape}};
+  for (const auto& c : choices)
+    if (key.compare(0, std::strlen(c.first), c.first) == 0)
+      return std::find(c.second->begin(), c.second->end(), pick) != c.second->end();
+  return false;
+}
 
 void tune_file_read(std::map<std::string, int>& picks) {
   const char* f = std::getenv("CUBE_AUTOTUNE_FILE");
@@ -90,7 +110,8 @@ void tune_file_read(std::map<std::string, int>& picks) {
   if (FILE* fp = std::fopen(f, "r")) {
     char k[256];
     int v;
-    while (std::fscanf(fp, "%255s %d", k, &v) == 2) picks[k] = v;
+    while (std::fscanf(fp, "%255s %d", k, &v) == 2)
+      if (tune_pick_valid(k, v)) picks[k] = v;
     std::fclose(fp);
   }
 }
@@ -429,7 +450,7 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
         // would; the FP64 form sums FP64 values in LDS atomics, whose order
         // (and last bits) follows the scheduling, and is not tuned
         const int thr = std::is_same<R, float>::value
-                            ? autotune(tune_key("dep_list_threads", level, n), {512, 256, 1024},
+                            ? autotune(tune_key("dep_list_threads", level, n), kTuneDepListThreads,
                                        [&](int t) {
                                          pm_deposit_lists<R>((int)g.izipx, L, boxes.p, nb, S, mesh, in.mass_p_cdm,
                                                              s, t);
@@ -452,7 +473,7 @@ void cube_p3m_ctx::run_level(int level, int slot, const std::vector<PMBox>& bx_i
       if (trials) ftune.ensure(sizeof(R) * (size_t)(nt * nreal));
       const R n3t = (R)((float)n * (float)n * (float)n);
       fshape = autotune(
-          key, {0, 1, 2},
+          key, kTuneFFTShape,
           [&](int sh) {
             fill_pattern((float*)ftune.p, nt * nreal, s);
             fft_convolve<R>(fc, (R*)ftune.p, cplx, G, nt, n3t, (R)0, s, nullptr, "", sh);
@@ -680,7 +701,7 @@ int cube_p3m_ctx::interp_shape(int level, int n, const PMLevel& L, const PMBox* 
   vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
   pmtune.ensure(1);
   return autotune(
-      key, {0, 1, 2},
+      key, kTuneInterpShape,
       [&](int sh) {
         CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
         zero_async(pmtune.p, sizeof(PMOut), s);
@@ -700,7 +721,7 @@ int cube_p3m_ctx::tune_pp(int iapm, const PPLevel& L, const PPBox* boxes, int nb
   vtune.ensure((size_t)std::max<int64_t>(np, 1) * 3);
   otune.ensure(1);
   return autotune(
-      tune_key("pp_depth", iapm, L.npgrid), {6, 4, 8, kPPBrick8 + 6, kPPBrick8 + 4, kPPBrick8 + 8},
+      tune_key("pp_depth", iapm, L.npgrid), kTunePPDepth,
       [&](int u) {
         CUBE_HIP(hipMemcpyAsync(vtune.p, vpc(), sizeof(float) * 3 * (size_t)np, hipMemcpyDeviceToDevice, s));
         zero_async(otune.p, sizeof(PPOut), s);
